@@ -1,0 +1,166 @@
+/*
+ * wilson.h -- C-ABI of the MI355X Wilson-Dslash / CG library (libwilson.so).
+ *
+ * Plain C: opaque handles, pointers and sizes.  The C++ operator API that
+ * mirrors the reference (lattice_fermion, lattice_gauge, DslashNew, CGinvert,
+ * ...) is a thin layer over these entry points, and foreign callers (ctypes,
+ * cgo, JNI) bind them directly.
+ *
+ * Conventions
+ *  - Host arrays use the reference layout ("ref" layout): a spinor field is
+ *    complex<double>[V*12] at eo(x)*12 + spin*3 + colour, a gauge field is four
+ *    complex<double>[V*9] arrays (one per direction x,y,z,t) at
+ *    eo(x)*9 + row*3 + col, with eo(x) = lex(x)/2 + parity(x)*V/2 and
+ *    lex(x) = x0 + L0*(x1 + L1*(x2 + L2*x3))  (include/operator.h:56-67).
+ *  - Device storage is owned by the library (even-odd, structure of arrays,
+ *    site index fastest).  Host buffers are owned by the caller.
+ *  - One wd_ctx = one device + one HIP stream; no internal locking (use one
+ *    context per host thread).  Every call is stream-ordered and returns only
+ *    when its results are complete, like the reference's blocking calls.
+ *  - Return codes: 0 success, < 0 error (see WD_E*), > 0 "not converged".
+ *    wd_strerror() names a code; wd_last_error() gives the context's last
+ *    message.
+ *  - `out` and `in` must be distinct fields (the reference does not support
+ *    aliasing either: DslashNew zeroes dest before reading src).
+ */
+#ifndef WILSON_H
+#define WILSON_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+#define WD_OK 0
+#define WD_EARG (-1)        /* invalid argument (shape, precision, null pointer ...) */
+#define WD_EHIP (-2)        /* a HIP runtime call failed */
+#define WD_ENOMEM (-3)      /* device or host allocation failed */
+#define WD_EBREAKDOWN (-4)  /* CG breakdown: non-finite or non-positive (p, A p) */
+#define WD_ENOTCONV 1       /* the solver stopped at its iteration limit */
+
+/* ---- precisions and link formats ---------------------------------------- */
+#define WD_F64 0   /* double */
+#define WD_F32 1   /* float */
+#define WD_F16 2   /* half (spinors: with one float scale per site) */
+
+#define WD_RECON_18 18 /* links stored as all 9 complex entries */
+#define WD_RECON_12 12 /* rows 0 and 1 stored; row 2 = conj(row0 x row1) (SU(3) only) */
+
+typedef struct wd_ctx wd_ctx;
+typedef struct wd_field wd_field;
+typedef struct wd_gauge wd_gauge;
+
+/* ---- context ------------------------------------------------------------ */
+/* L = {Lx, Ly, Lz, Lt}, every extent even and >= 2.  bc[mu] = +1 periodic,
+ * -1 antiperiodic (NULL = periodic everywhere, the reference's only boundary
+ * condition, src/dslash_new.cpp:1847).  device = HIP ordinal.
+ * Replaces the reference's lattice set-up in src/main.cpp:17-66. */
+int wd_create(wd_ctx **ctx, const int L[4], const int bc[4], int device);
+void wd_destroy(wd_ctx *ctx);
+const char *wd_strerror(int code);
+const char *wd_last_error(const wd_ctx *ctx);
+/* Blocks until all work on the context's stream is complete. */
+int wd_sync(wd_ctx *ctx);
+/* Volume V = Lx*Ly*Lz*Lt of the context. */
+long wd_volume(const wd_ctx *ctx);
+
+/* ---- gauge field (lattice_gauge, include/lattice_gauge.h:11-26) ---------- */
+/* prec: WD_F64 | WD_F32 | WD_F16 ; recon: WD_RECON_18 | WD_RECON_12. */
+int wd_gauge_create(wd_ctx *ctx, wd_gauge **U, int prec, int recon);
+/* A[mu] = complex<double>[V*9] in ref layout (the reference's U.A[mu]). */
+int wd_gauge_upload_ref(wd_ctx *ctx, wd_gauge *U, const double *const A[4]);
+void wd_gauge_destroy(wd_gauge *U);
+
+/* ---- spinor fields (lattice_fermion, include/lattice_fermion.h:12-35) ---- */
+/* A field holds nrhs full-lattice spinors (both parities) in precision prec.
+ * Fresh fields are zero. */
+int wd_field_create(wd_ctx *ctx, wd_field **f, int prec, int nrhs);
+void wd_field_destroy(wd_field *f);
+/* A = complex<double>[V*12] in ref layout (the reference's lattice_fermion.A). */
+int wd_field_upload_ref(wd_ctx *ctx, wd_field *f, const double *A, int rhs);
+int wd_field_download_ref(wd_ctx *ctx, const wd_field *f, double *A, int rhs);
+/* f = 0 (lattice_fermion::clean, src/lattice_fermion.cpp:44-49). */
+int wd_field_zero(wd_ctx *ctx, wd_field *f);
+/* dst = src, converting precision when the two differ (same nrhs). */
+int wd_field_copy(wd_ctx *ctx, wd_field *dst, const wd_field *src);
+
+/* ---- operator (F64 fields and an F64/18 gauge) --------------------------- */
+/* Hopping term: out[cb] = -1/2 sum_mu [(1 - f g_mu) U_mu(x) in(x+mu)
+ *   + (1 + f g_mu) U_mu^dag(x-mu) in(x-mu)] for sites of parity cb, read from
+ * parity 1-cb; out[1-cb] = 0.  f = -1 when dagger.
+ * Replaces DslashoffdNew (src/dslash_new.cpp:1341-2677) / Dslashoffd
+ * (src/dslash.cpp:38-1249). */
+int wd_hop(wd_ctx *ctx, wd_field *out, const wd_field *in, const wd_gauge *U, int dagger, int cb);
+/* out = (4 + mass) in over the full lattice.
+ * Replaces DslashEEOONew (src/dslash_new.cpp:1301-1338); DslashEE / DslashOO
+ * (src/dslash.cpp:11-35) are the two halves of it. */
+int wd_diag(wd_ctx *ctx, wd_field *out, const wd_field *in, double mass);
+/* out = M in (dagger = 0) or M^dagger in (dagger = 1), one fused pass.
+ * Replaces DslashNew (src/invert.cpp:138-156) and Dslash (src/invert.cpp:123-136). */
+int wd_apply_M(wd_ctx *ctx, wd_field *out, const wd_field *in, const wd_gauge *U, double mass,
+               int dagger);
+/* out = M^dagger M in; tmp is caller scratch of the same shape.
+ * The normal operator of src/invert.cpp:79-80. */
+int wd_apply_MdagM(wd_ctx *ctx, wd_field *out, const wd_field *in, wd_field *tmp,
+                   const wd_gauge *U, double mass);
+
+/* ---- reductions (include/operator_mpi.h:15-79) ---------------------------- */
+/* *out = sum |f_i|^2 over all 12V complex entries (norm_2). */
+int wd_norm2(wd_ctx *ctx, const wd_field *f, double *out);
+/* out = sum conj(a_i) b_i (vector_p), out[0] real, out[1] imaginary part. */
+int wd_dot(wd_ctx *ctx, const wd_field *a, const wd_field *b, double out[2]);
+
+/* ---- solvers ------------------------------------------------------------ */
+typedef struct wd_cg_params {
+    int max_iter;        /* the reference's `max`: the loop runs f = 1 .. max_iter-1 */
+    double tol;          /* stop threshold on |r| */
+    int tol_relative;    /* 0: |r| < tol (the reference's absolute test, src/invert.cpp:108);
+                            1: |r| < tol * |M^dagger b| */
+    int use_x0;          /* 1: start from the incoming x (the reference seeds it with rand(),
+                            src/invert.cpp:42-44); 0: start from x = 0 */
+    int check_every;     /* iterations launched between host-side convergence checks (>= 1) */
+    int use_graph;       /* 1: replay one captured hipGraph per check_every iterations */
+    double *rsd_history; /* optional host array [max_iter]: |r| after each iteration */
+} wd_cg_params;
+
+typedef struct wd_cg_report {
+    int iterations;      /* f at which the stop test passed (or the last f run) */
+    int converged;       /* 1 when the stop test passed */
+    double rsd;          /* |r| of the recursive residual at exit */
+    double true_rsd;     /* |M^dagger b - M^dagger M x| recomputed in FP64 at exit */
+    double rhs_norm;     /* |M^dagger b| */
+    double seconds;      /* wall time of the solve on the device stream */
+    long operator_applies; /* applications of M or M^dagger */
+} wd_cg_report;
+
+/* Fills params with the reference's settings: max 500, absolute tol 1e-9
+ * (src/main.cpp:47-48), use_x0 = 1, check_every 8, use_graph 1. */
+void wd_cg_params_default(wd_cg_params *p);
+
+/* CGNR on M^dagger M x = M^dagger b, FP64 throughout, the reference's algorithm
+ * (complex alpha and beta, same recurrences, same stop test).  x (F64) holds x0
+ * on entry when params->use_x0, the solution on return; b is F64.
+ * Replaces CGinvert (src/invert.cpp:15-121).  Returns 0, WD_ENOTCONV or < 0. */
+int wd_cg(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd_gauge *U, double mass,
+          const wd_cg_params *params, wd_cg_report *report);
+
+/* ---- measurement and tuning --------------------------------------------------
+ * No counterpart in the reference, which times whole phases with MPI_Wtime
+ * (src/main.cpp:88-110). */
+#define WD_OP_HOP 0 /* one-parity hop (wd_hop without zeroing the other half) */
+#define WD_OP_M 1   /* fused M / M^dagger over both parities (wd_apply_M) */
+/* Launches the operator `iters` times back to back on the context stream and
+ * returns the average launch duration measured with HIP events on that
+ * stream. */
+int wd_time_op(wd_ctx *ctx, int op, wd_field *out, const wd_field *in, const wd_gauge *U,
+               double mass, int dagger, int cb, int iters, double *ms_per_launch);
+
+#define WD_OPT_XCD_REMAP 1 /* 1 (default): each XCD processes a contiguous site range */
+int wd_set_option(wd_ctx *ctx, int option, int value);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,278 @@
+// wd_hop_f64.hip -- FP64 Wilson hopping term and fused M / M^dagger kernels.
+//
+// One thread per output site.  For each of the 8 neighbours the thread loads
+// the full neighbour spinor (12 complex, coalesced across the wavefront), the
+// link (9 complex, streamed with non-temporal loads because every link is
+// read exactly once per half-lattice pass), spin-projects to a half spinor,
+// multiplies by U or U^dagger and reconstructs into a register accumulator.
+// The -1/2 of the hopping term (src/dslash.cpp:552 `* half`, with the minus of
+// `tmp = -(...)`) is applied once in the epilogue; scaling by 0.5 is exact.
+//
+// Modes (template):
+//   HOP_ONLY  out[p] = -1/2 hop(in)                       DslashoffdNew
+//   HOP_M     out[p] = a * same[p] - 1/2 hop(in)          DslashNew, a = 4 + m
+//   HOP_M_DOT HOP_M plus a per-workgroup partial of sum conj(dotv) * out, the
+//             (p, q) product of the CG step fused into q = M^dagger (M p).
+#include "wd_dirac.h"
+#include "wd_internal.h"
+
+namespace {
+
+typedef double d2v __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ cpx<double> ld(const double2 *__restrict__ p, unsigned idx)
+{
+    const d2v v = *reinterpret_cast<const d2v *>(p + idx);
+    return {v.x, v.y};
+}
+
+__device__ __forceinline__ cpx<double> ld_nt(const double2 *__restrict__ p, unsigned idx)
+{
+    const d2v v = __builtin_nontemporal_load(reinterpret_cast<const d2v *>(p + idx));
+    return {v.x, v.y};
+}
+
+__device__ __forceinline__ void st(double2 *__restrict__ p, long idx, cpx<double> z)
+{
+    d2v v;
+    v.x = z.r;
+    v.y = z.i;
+    *reinterpret_cast<d2v *>(p + idx) = v;
+}
+
+__device__ __forceinline__ void load_spinor(const double2 *__restrict__ base, int stride, int j,
+                                            cpx<double> (&psi)[4][3])
+{
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            psi[s][c] = ld(base + (size_t)(s * 3 + c) * (unsigned)stride, (unsigned)j);
+}
+
+__device__ __forceinline__ void load_link(const double2 *__restrict__ base, int stride, int j,
+                                          cpx<double> (&U)[3][3])
+{
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+            U[a][b] = ld_nt(base + (size_t)(a * 3 + b) * (unsigned)stride, (unsigned)j);
+}
+
+// Forces the accumulator to be materialised here so the compiler cannot
+// defer the arithmetic of a direction (and keep its loads live) past this point.
+__device__ __forceinline__ void pin_acc(cpx<double> (&acc)[4][3])
+{
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            asm volatile("" : "+v"(acc[s][c].r), "+v"(acc[s][c].i));
+}
+
+// One direction pair (forward and backward) of the hop.
+template <int MU, bool DAG>
+__device__ __forceinline__ void hop_mu(cpx<double> (&acc)[4][3], const double2 *__restrict__ in_o,
+                                       const double2 *__restrict__ U, const Geom &g, int p, int i,
+                                       int jf, int jb, double sf, double sb)
+{
+    const int stride = g.stride;
+    {
+        cpx<double> psi[4][3], W[3][3], h[2][3], r[2][3];
+        load_spinor(in_o, stride, jf, psi);
+        load_link(U + (long)((MU * 2 + p) * 9) * stride, stride, i, W);
+        using D = Dir<MU, true, DAG>;
+        project<D>(psi, h);
+        su3_mul_half<false>(W, h, r);
+        reconstruct<D>(acc, r, sf);
+    }
+    pin_acc(acc);
+    {
+        cpx<double> psi[4][3], W[3][3], h[2][3], r[2][3];
+        load_spinor(in_o, stride, jb, psi);
+        load_link(U + (long)((MU * 2 + (1 - p)) * 9) * stride, stride, jb, W);
+        using D = Dir<MU, false, DAG>;
+        project<D>(psi, h);
+        su3_mul_half<true>(W, h, r);
+        reconstruct<D>(acc, r, sb);
+    }
+    pin_acc(acc);
+}
+
+template <bool DAG, int MODE>
+__global__ __launch_bounds__(kBlock) void k_hop_f64(double2 *__restrict__ out,
+                                                    const double2 *__restrict__ in,
+                                                    const double2 *__restrict__ U, Geom g,
+                                                    int cb_first, int nparity, double a,
+                                                    const double2 *__restrict__ same,
+                                                    const double2 *__restrict__ dotv,
+                                                    double2 *__restrict__ partials, int remap,
+                                                    const int *__restrict__ stop)
+{
+    // solver loops launch ahead of their convergence check; a set stop flag
+    // turns the remaining launches into no-ops
+    if (stop && *stop)
+        return;
+    const int lb = logical_block(remap);
+    const int bpp = gridDim.x / nparity; // workgroups per parity
+    const int p = (nparity == 2) ? lb / bpp : cb_first;
+    const int i = (lb - (nparity == 2 ? p * bpp : 0)) * kBlock + threadIdx.x;
+    const bool active = i < g.Vh;
+
+    cpx<double> res[4][3];
+    if (active) {
+        // coordinates of site i of parity p
+        const int Lx = 2 * g.Lxh;
+        int rr = i / g.Lxh;
+        const int xh = i - rr * g.Lxh;
+        const int y = rr % g.Ly;
+        rr /= g.Ly;
+        const int z = rr % g.Lz;
+        const int t = rr / g.Lz;
+        const int x = 2 * xh + ((p + y + z + t) & 1);
+        const int row = i - xh; // index of (xh = 0, y, z, t)
+
+        const double2 *in_o = in + (long)(1 - p) * 12 * g.stride;
+        cpx<double> acc[4][3];
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                acc[s][c] = {0.0, 0.0};
+
+        {
+            const int xf = (x + 1 == Lx) ? 0 : x + 1, xb = (x == 0) ? Lx - 1 : x - 1;
+            hop_mu<0, DAG>(acc, in_o, U, g, p, i, row + (xf >> 1), row + (xb >> 1),
+                           x == Lx - 1 ? g.bc[0] : 1.0, x == 0 ? g.bc[0] : 1.0);
+        }
+        {
+            const int s = g.Lxh;
+            const int jf = (y + 1 == g.Ly) ? i - (g.Ly - 1) * s : i + s;
+            const int jb = (y == 0) ? i + (g.Ly - 1) * s : i - s;
+            hop_mu<1, DAG>(acc, in_o, U, g, p, i, jf, jb, y == g.Ly - 1 ? g.bc[1] : 1.0,
+                           y == 0 ? g.bc[1] : 1.0);
+        }
+        {
+            const int s = g.Lxh * g.Ly;
+            const int jf = (z + 1 == g.Lz) ? i - (g.Lz - 1) * s : i + s;
+            const int jb = (z == 0) ? i + (g.Lz - 1) * s : i - s;
+            hop_mu<2, DAG>(acc, in_o, U, g, p, i, jf, jb, z == g.Lz - 1 ? g.bc[2] : 1.0,
+                           z == 0 ? g.bc[2] : 1.0);
+        }
+        {
+            const int s = g.Lxh * g.Ly * g.Lz;
+            const int jf = (t + 1 == g.Lt) ? i - (g.Lt - 1) * s : i + s;
+            const int jb = (t == 0) ? i + (g.Lt - 1) * s : i - s;
+            hop_mu<3, DAG>(acc, in_o, U, g, p, i, jf, jb, t == g.Lt - 1 ? g.bc[3] : 1.0,
+                           t == 0 ? g.bc[3] : 1.0);
+        }
+
+        double2 *out_p = out + (long)p * 12 * g.stride;
+        if constexpr (MODE == HOP_ONLY) {
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    res[s][c] = {-0.5 * acc[s][c].r, -0.5 * acc[s][c].i};
+                    st(out_p, (long)(s * 3 + c) * g.stride + i, res[s][c]);
+                }
+        } else {
+            const double2 *same_p = same + (long)p * 12 * g.stride;
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const cpx<double> v = ld(same_p + (size_t)(s * 3 + c) * g.stride, (unsigned)i);
+                    res[s][c] = {fma(a, v.r, -0.5 * acc[s][c].r), fma(a, v.i, -0.5 * acc[s][c].i)};
+                    st(out_p, (long)(s * 3 + c) * g.stride + i, res[s][c]);
+                }
+        }
+    }
+
+    if constexpr (MODE == HOP_M_DOT) {
+        // partial of sum conj(dotv) * out for this workgroup, fixed order
+        double pr = 0.0, pi = 0.0;
+        if (active) {
+            const double2 *dv = dotv + (long)p * 12 * g.stride;
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const cpx<double> v = ld(dv + (size_t)(s * 3 + c) * g.stride, (unsigned)i);
+                    pr = fma(v.r, res[s][c].r, pr);
+                    pr = fma(v.i, res[s][c].i, pr);
+                    pi = fma(v.r, res[s][c].i, pi);
+                    pi = fma(-v.i, res[s][c].r, pi);
+                }
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            pr += __shfl_xor(pr, off, 64);
+            pi += __shfl_xor(pi, off, 64);
+        }
+        __shared__ double sr[kBlock / 64], si[kBlock / 64];
+        const int w = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0) {
+            sr[w] = pr;
+            si[w] = pi;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double tr = 0.0, ti = 0.0;
+            for (int k = 0; k < kBlock / 64; ++k) {
+                tr += sr[k];
+                ti += si[k];
+            }
+            partials[lb] = make_double2(tr, ti);
+        }
+    }
+}
+
+template <bool DAG>
+int launch_mode(wd_ctx *ctx, double2 *out, const double2 *in, const double2 *U, int cb, int mode,
+                double a, const double2 *same, const double2 *dotv, double2 *partials,
+                const int *stop)
+{
+    const Geom &g = ctx->geom;
+    const int nparity = (cb < 0) ? 2 : 1;
+    const int bpp = (g.Vh + kBlock - 1) / kBlock;
+    const int grid = bpp * nparity;
+    const int cb_first = (cb < 0) ? 0 : cb;
+    if (mode == HOP_M_DOT && grid > ctx->max_partials)
+        return wd_set_error(ctx, WD_EARG, "hop: too many workgroups for the partials buffer");
+    switch (mode) {
+    case HOP_ONLY:
+        hipLaunchKernelGGL((k_hop_f64<DAG, HOP_ONLY>), dim3(grid), dim3(kBlock), 0, ctx->stream, out,
+                           in, U, g, cb_first, nparity, a, same, dotv, partials, ctx->xcd_remap, stop);
+        break;
+    case HOP_M:
+        hipLaunchKernelGGL((k_hop_f64<DAG, HOP_M>), dim3(grid), dim3(kBlock), 0, ctx->stream, out,
+                           in, U, g, cb_first, nparity, a, same, dotv, partials, ctx->xcd_remap, stop);
+        break;
+    default:
+        hipLaunchKernelGGL((k_hop_f64<DAG, HOP_M_DOT>), dim3(grid), dim3(kBlock), 0, ctx->stream,
+                           out, in, U, g, cb_first, nparity, a, same, dotv, partials,
+                           ctx->xcd_remap, stop);
+        break;
+    }
+    return wd_check_launch(ctx, "k_hop_f64");
+}
+
+} // namespace
+
+// cb in {0, 1}: one output parity; cb = -1: both parities in one launch.
+int wd_launch_hop_f64(wd_ctx *ctx, double2 *out, const double2 *in, const double2 *U, int dagger,
+                      int cb, int mode, double a, const double2 *same, const double2 *dotv,
+                      double2 *partials, const int *stop)
+{
+    if (dagger)
+        return launch_mode<true>(ctx, out, in, U, cb, mode, a, same, dotv, partials, stop);
+    return launch_mode<false>(ctx, out, in, U, cb, mode, a, same, dotv, partials, stop);
+}
+
+// Number of workgroups a hop launch over `nparity` parities uses (partials count).
+int wd_hop_grid(const wd_ctx *ctx, int nparity)
+{
+    return ((ctx->geom.Vh + kBlock - 1) / kBlock) * nparity;
+}

@@ -1,0 +1,125 @@
+// wd_internal.h -- shared definitions of the HIP library behind include/wilson.h.
+//
+// Device storage (all fields even-odd checkerboarded, structure of arrays):
+//   F64 spinor : double2 [rhs][parity][12][stride]        (k = spin*3 + colour)
+//   F32 spinor : float2  [rhs][parity][12][stride]
+//   F16 spinor : half2   [rhs][parity][12][stride] + float scale [rhs][parity][stride]
+//   gauge      : T2      [mu][parity][nrow*3][stride]     (row-major 3x3, nrow = 3 or 2)
+// with stride = Vh rounded up to a multiple of 64, site index i fastest, so a
+// wavefront reading component k of 64 consecutive sites issues one fully
+// coalesced 1 KiB (F64) / 512 B (F32) request.  Padding sites are zero.
+//
+// Position i inside a parity block is the reference's eo index within that
+// block: i = xh + Lxh*(y + Ly*(z + Lz*t)), x = 2*xh + ((parity+y+z+t) & 1)
+// (include/operator.h:62-67).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "../../include/wilson.h"
+
+struct Geom {
+    int Lxh, Ly, Lz, Lt; // half x extent and the other three extents
+    int Vh;              // sites per parity
+    int stride;          // component stride (>= Vh, multiple of 64)
+    double bc[4];        // boundary phase applied to hops that cross the boundary in mu
+};
+
+struct wd_ctx {
+    int L[4];
+    int bc[4];
+    int device;
+    long V, Vh, stride;
+    Geom geom;
+    hipStream_t stream;
+    // reduction partials and solver scratch
+    void *partials;       // double2 [max_partials]
+    int max_partials;
+    void *dscalars;       // small device scratch for solver state
+    size_t dscalars_bytes;
+    void *hpinned;        // pinned host scratch (4 KiB)
+    // host<->device staging for layout conversion
+    void *staging;
+    size_t staging_bytes;
+    int xcd_remap;        // 1: map workgroups so each XCD owns a contiguous site range
+    std::string last_error;
+};
+
+struct wd_field {
+    wd_ctx *ctx;
+    int prec;
+    int nrhs;
+    void *data;           // element type by prec
+    float *scale;         // F16 only: per-site scale [rhs][parity][stride]
+    size_t rhs_elems;     // elements (complex) per rhs = 24 * stride
+    size_t bytes;
+};
+
+struct wd_gauge {
+    wd_ctx *ctx;
+    int prec;
+    int recon;
+    void *data;
+    size_t bytes;
+};
+
+// ---- error plumbing ---------------------------------------------------------
+int wd_set_error(wd_ctx *ctx, int code, const std::string &msg);
+
+#define WD_HIP_CHECK(ctx, expr)                                                                    \
+    do {                                                                                           \
+        hipError_t _e = (expr);                                                                    \
+        if (_e != hipSuccess)                                                                      \
+            return wd_set_error((ctx), WD_EHIP,                                                    \
+                                std::string(#expr) + ": " + hipGetErrorString(_e));                \
+    } while (0)
+
+#define WD_ARG_CHECK(ctx, cond, msg)                                                               \
+    do {                                                                                           \
+        if (!(cond))                                                                               \
+            return wd_set_error((ctx), WD_EARG, (msg));                                            \
+    } while (0)
+
+// Checks the launch status of the last kernel.
+int wd_check_launch(wd_ctx *ctx, const char *what);
+
+// ---- launch helpers ----------------------------------------------------------
+constexpr int kBlock = 256;          // threads per workgroup for site kernels
+constexpr int kRedBlocks = 1024;     // workgroups of the BLAS-1 / reduction kernels
+constexpr int kNumXCD = 8;           // MI355X: 8 accelerator complex dies
+
+// Hardware hands workgroup b to XCD (b mod 8).  With remap on, logical block
+// (b mod 8) * (n/8) + b/8 is processed instead, so each XCD (and its L2) owns
+// one contiguous range of sites and neighbour spinors are reused within it.
+__device__ __forceinline__ int logical_block(int remap)
+{
+    const int b = blockIdx.x, n = gridDim.x;
+    if (remap && (n % kNumXCD) == 0)
+        return (b % kNumXCD) * (n / kNumXCD) + b / kNumXCD;
+    return b;
+}
+
+// ---- precision helpers ---------------------------------------------------------
+inline size_t prec_bytes_complex(int prec)
+{
+    return prec == WD_F64 ? 16 : prec == WD_F32 ? 8 : 4;
+}
+
+// ---- internal entry points shared between translation units --------------------
+int wd_launch_hop_f64(wd_ctx *ctx, double2 *out, const double2 *in, const double2 *U, int dagger,
+                      int cb, int mode, double a, const double2 *same, const double2 *dotv,
+                      double2 *partials, const int *stop);
+int wd_launch_zero(wd_ctx *ctx, void *p, size_t bytes);
+int wd_ensure_staging(wd_ctx *ctx, size_t bytes);
+
+// hop modes for the fused FP64 kernel
+enum HopMode {
+    HOP_ONLY = 0,     // out[p] = -1/2 hop(in)
+    HOP_M = 1,        // out[p] = a * same[p] - 1/2 hop(in)  (same == in for M)
+    HOP_M_DOT = 2,    // as HOP_M, plus per-block partial of sum conj(dotv) * out
+};
