@@ -1,0 +1,87 @@
+// wd_runtime.cpp -- context cache and RAII wrappers over libwilson.so.
+#include "wd_runtime.h"
+
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+
+namespace wdrt {
+
+namespace {
+std::mutex g_mu;
+std::map<std::tuple<int, int, int, int>, wd_ctx *> g_ctx;
+} // namespace
+
+void check(wd_ctx *ctx, int rc, const char *what)
+{
+    if (rc < 0)
+        throw std::runtime_error(std::string(what) + ": " + wd_strerror(rc) + ": " +
+                                 (ctx ? wd_last_error(ctx) : ""));
+}
+
+wd_ctx *context(const int *subgs, const int *site_vec)
+{
+    for (int mu = 0; mu < 4; ++mu)
+        if (subgs[mu] != site_vec[mu])
+            throw std::runtime_error(
+                "the lattice is partitioned (subgs != site_vec): one GPU holds the whole "
+                "lattice in this build");
+    std::lock_guard<std::mutex> lock(g_mu);
+    const auto key = std::make_tuple(site_vec[0], site_vec[1], site_vec[2], site_vec[3]);
+    auto it = g_ctx.find(key);
+    if (it != g_ctx.end())
+        return it->second;
+    const char *dev = std::getenv("WD_DEVICE");
+    wd_ctx *c = nullptr;
+    const int rc = wd_create(&c, site_vec, nullptr, dev ? std::atoi(dev) : 0);
+    if (rc != WD_OK)
+        throw std::runtime_error(std::string("wd_create: ") + wd_strerror(rc));
+    g_ctx[key] = c;
+    return c;
+}
+
+Field::Field(wd_ctx *c, int prec) : ctx(c)
+{
+    check(ctx, wd_field_create(ctx, &f, prec, 1), "wd_field_create");
+}
+
+Field::Field(wd_ctx *c, const lattice_fermion &src) : Field(c)
+{
+    upload(src);
+}
+
+Field::~Field()
+{
+    wd_field_destroy(f);
+}
+
+void Field::upload(const lattice_fermion &src)
+{
+    check(ctx, wd_field_upload_ref(ctx, f, reinterpret_cast<const double *>(src.A), 0),
+          "wd_field_upload_ref");
+}
+
+void Field::download(lattice_fermion &dst) const
+{
+    check(ctx, wd_field_download_ref(ctx, f, reinterpret_cast<double *>(dst.A), 0),
+          "wd_field_download_ref");
+}
+
+Gauge::Gauge(wd_ctx *c, const lattice_gauge &U) : ctx(c)
+{
+    check(ctx, wd_gauge_create(ctx, &g, WD_F64, WD_RECON_18), "wd_gauge_create");
+    const double *A[4];
+    for (int mu = 0; mu < 4; ++mu)
+        A[mu] = reinterpret_cast<const double *>(U.A[mu]);
+    check(ctx, wd_gauge_upload_ref(ctx, g, A), "wd_gauge_upload_ref");
+}
+
+Gauge::~Gauge()
+{
+    wd_gauge_destroy(g);
+}
+
+} // namespace wdrt

@@ -198,7 +198,7 @@ def test_zero_links_closed_form(w):
         res = ctx.cg(x, fb, U, mass, max_iter=10, tol=1e-12, tol_relative=True, use_x0=False)
         xs = x.download()
     assert res["iterations"] == 1
-    assert rel_l2(xs, b / (4 + mass)) < 1e-15
+    assert rel_l2(xs, b / (4 + mass)) < 1e-14
 
 
 def test_argument_errors(w):
