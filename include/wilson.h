@@ -146,6 +146,55 @@ void wd_cg_params_default(wd_cg_params *p);
 int wd_cg(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd_gauge *U, double mass,
           const wd_cg_params *params, wd_cg_report *report);
 
+/* ---- mixed precision and many right-hand sides ----------------------------------
+ * No counterpart in the reference (its only solver is the FP64 CGinvert,
+ * src/invert.cpp:25-121).  The inner iterations use FP32 arithmetic on
+ * F16 (per-site scaled) or F32 spinors and a sloppy gauge copy (F32 18/12-real
+ * or F16 12-real); FP64 is kept for reductions, the solution and the true
+ * residual of the reliable updates. */
+typedef struct wd_mixed_params {
+    int max_iter;       /* inner iterations per system */
+    double tol;         /* stop when the FP64 |M^dag b - M^dag M x| < tol |M^dag b| */
+    double delta;       /* reliable update when the sloppy |r| < delta x |r| at the last one */
+    int spinor_prec;    /* WD_F16 or WD_F32 storage of the inner vectors */
+    int rhs_per_thread; /* 1, 2 or 4 systems per thread; 0 = autotune */
+    int check_every;    /* iterations between host polls of the convergence flags */
+    int use_graph;      /* replay each block of check_every iterations as one hipGraph */
+} wd_mixed_params;
+
+typedef struct wd_mixed_report {
+    int iterations;       /* inner iterations */
+    int reliable_updates; /* FP64 true-residual recomputations */
+    int converged;        /* 1 when the true residual met tol */
+    double true_rel_rsd;  /* |M^dag b - M^dag M x| / |M^dag b| at the last reliable update */
+    double rhs_norm;      /* |M^dag b| */
+    int rhs_per_thread;   /* the rhs-per-thread the operator ran with */
+} wd_mixed_report;
+
+/* max_iter 100000, tol 1e-10, delta 1e-2, F16 spinors, autotuned rhs per
+ * thread, check_every 16, graphs on. */
+void wd_mixed_params_default(wd_mixed_params *p);
+
+/* Solves M_r^dag M_r x_r = M_r^dag b_r for all nrhs systems of b at once
+ * (x, b: F64 fields with the same nrhs; masses[r] = m_r, host array), x_r
+ * starting from 0.  U: F64 18-real gauge (reliable updates); Us: the sloppy
+ * copy of the same links.  reports (host, nrhs entries) and seconds may be
+ * NULL.  Returns 0 when every system converged, WD_ENOTCONV, or < 0. */
+int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd_gauge *U,
+                const wd_gauge *Us, const double *masses, const wd_mixed_params *params,
+                wd_mixed_report *reports, double *seconds);
+
+/* out_r = M_r in_r (dagger 0) or M_r^dag in_r with the sloppy operator for all
+ * nrhs systems (fields F16 or F32 of the same format, masses[r] host array),
+ * rhs_per_thread 1, 2 or 4. */
+int wd_apply_M_sloppy(wd_ctx *ctx, wd_field *out, const wd_field *in, const wd_gauge *Us,
+                      const double *masses, int dagger, int rhs_per_thread);
+
+/* Autotuner: times the sloppy M over f's nrhs systems for 1, 2 and 4 systems
+ * per thread (ms[0..2], HIP events on the context stream) and records the
+ * fastest (*best_R) for later solves of that shape. */
+int wd_tune_report(wd_ctx *ctx, const wd_field *f, const wd_gauge *Us, int *best_R, double ms[3]);
+
 /* ---- measurement and tuning --------------------------------------------------
  * No counterpart in the reference, which times whole phases with MPI_Wtime
  * (src/main.cpp:88-110). */

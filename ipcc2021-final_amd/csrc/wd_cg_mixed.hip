@@ -1,0 +1,738 @@
+// wd_cg_mixed.hip -- mixed-precision CG with reliable updates on many
+// right-hand sides (several masses allowed), M^dagger M x = M^dagger b.
+//
+// Inner iterations run on the FP32-arithmetic operator with F16 (or F32)
+// spinor storage and compressed links (wd_hop_sloppy.hip).  FP64 is kept for
+// the global reductions, the solution (x += x_sloppy at every reliable update)
+// and the true residual r = M^dagger b - M^dagger M x, recomputed with the
+// FP64 operator whenever the sloppy residual has dropped by `delta` since the
+// last update (reliable updates: Sleijpen & van der Vorst, "Reliable updated
+// residuals in hybrid Bi-CG methods", Computing 56 (1996); the form used for
+// lattice QCD by Clark et al., Comput. Phys. Commun. 181 (2010) 1517).  A
+// system has converged when the FP64 true residual is below tol * |M^dagger b|.
+//
+// Every system carries its own scalars and flags in device memory; one
+// iteration is a fixed sequence of launches in which converged systems and
+// systems without a pending reliable update are skipped by whole workgroups,
+// so the host only polls the flags every `check_every` iterations and the
+// sequence can be replayed from a captured hipGraph:
+//   xpay    p = r_s + beta p
+//   M       qq = M p                      (sloppy, all active systems in one launch)
+//   M^dag   q = M^dag qq, partial (p, q)  (sloppy)
+//   alpha   alpha = rho / (p, q)
+//   update  x_s += alpha p, r_s -= alpha q, partial |r_s|^2
+//   decide  reliable update due?  else beta = |r_s|^2 / rho, rho = |r_s|^2
+//   relx    x += x_s, x_s = 0                             (pending systems)
+//   M, M^dag (FP64) on x                                   (pending systems)
+//   relr    r = M^dag b - M^dag M x -> r_s, partial |r|^2 (pending systems)
+//   relfin  converged?  else beta = |r|^2 / rho, rho = |r|^2
+//   alldone stop flag when no system is active
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <map>
+#include <tuple>
+#include <vector>
+
+#include "wd_blas.h"
+#include "wd_fields.h"
+#include "wd_internal.h"
+#include "wd_sloppy.h"
+
+using namespace wdf;
+
+namespace {
+
+struct MxState {
+    double rho;      // |r|^2 carried by the recursion
+    double alpha, beta;
+    double maxr;     // |r| at the last reliable update
+    double target;   // tol * |M^dag b|
+    double rsd;      // last sloppy |r_s|
+    double true_rsd; // |M^dag b - M^dag M x| at the last reliable update
+    double b_norm;   // |M^dag b|
+    int iter, status, updates, pad; // status: 0 running, 1 converged, 2 breakdown, 3 limit
+};
+
+__device__ __forceinline__ double2 sum_rhs_partials(const double2 *partials, int count)
+{
+    return sum_partials_1024(partials + (size_t)blockIdx.x * count, count);
+}
+
+// ---- elementwise kernels, blockIdx.y = rhs, one thread per (parity, site) ----
+template <int SF>
+__global__ __launch_bounds__(kBlock) void k_mx_xpay(char *__restrict__ p, float *__restrict__ ps,
+                                                    const char *__restrict__ r,
+                                                    const float *__restrict__ rs,
+                                                    size_t rhs_bytes, int stride,
+                                                    const MxState *__restrict__ st,
+                                                    const int *__restrict__ active)
+{
+    const int rhs = blockIdx.y;
+    if (!active[rhs])
+        return;
+    const int e = blockIdx.x * kBlock + threadIdx.x;
+    if (e >= 2 * stride)
+        return;
+    const int par = e / stride, i = e - par * stride;
+    const size_t off = rhs * rhs_bytes + (size_t)par * spinor_chunks(SF) * 16 * stride;
+    const size_t so = ((size_t)rhs * 2 + par) * stride;
+    const float beta = (float)st[rhs].beta;
+    cpx<float> vp[4][3], vr[4][3];
+    Spinor<SF>::load(p + off, ps ? ps + so : nullptr, stride, (unsigned)i, vp);
+    Spinor<SF>::load(r + off, rs ? rs + so : nullptr, stride, (unsigned)i, vr);
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            vp[s][c] = {fmaf(beta, vp[s][c].r, vr[s][c].r), fmaf(beta, vp[s][c].i, vr[s][c].i)};
+    Spinor<SF>::store(p + off, ps ? ps + so : nullptr, stride, (unsigned)i, vp);
+}
+
+template <int SF>
+__global__ __launch_bounds__(kBlock) void k_mx_update(
+    char *__restrict__ xs, char *__restrict__ r, float *__restrict__ rs,
+    const char *__restrict__ p, const float *__restrict__ ps, const char *__restrict__ q,
+    const float *__restrict__ qs, size_t rhs_bytes, size_t xs_rhs_bytes, int stride,
+    const MxState *__restrict__ st, const int *__restrict__ active, double2 *__restrict__ partials)
+{
+    const int rhs = blockIdx.y;
+    if (!active[rhs])
+        return;
+    const int e = blockIdx.x * kBlock + threadIdx.x;
+    double acc = 0.0;
+    if (e < 2 * stride) {
+        const int par = e / stride, i = e - par * stride;
+        const size_t off = rhs * rhs_bytes + (size_t)par * spinor_chunks(SF) * 16 * stride;
+        const size_t xoff = rhs * xs_rhs_bytes + (size_t)par * spinor_chunks(WD_F32) * 16 * stride;
+        const size_t so = ((size_t)rhs * 2 + par) * stride;
+        const float alpha = (float)st[rhs].alpha;
+        cpx<float> vx[4][3], vr[4][3], vp[4][3], vq[4][3];
+        Spinor<WD_F32>::load(xs + xoff, nullptr, stride, (unsigned)i, vx);
+        Spinor<SF>::load(r + off, rs ? rs + so : nullptr, stride, (unsigned)i, vr);
+        Spinor<SF>::load(p + off, ps ? ps + so : nullptr, stride, (unsigned)i, vp);
+        Spinor<SF>::load(q + off, qs ? qs + so : nullptr, stride, (unsigned)i, vq);
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                vx[s][c] = {fmaf(alpha, vp[s][c].r, vx[s][c].r),
+                            fmaf(alpha, vp[s][c].i, vx[s][c].i)};
+                vr[s][c] = {fmaf(-alpha, vq[s][c].r, vr[s][c].r),
+                            fmaf(-alpha, vq[s][c].i, vr[s][c].i)};
+            }
+        Spinor<WD_F32>::store(xs + xoff, nullptr, stride, (unsigned)i, vx);
+        Spinor<SF>::store(r + off, rs ? rs + so : nullptr, stride, (unsigned)i, vr);
+        // |r_s|^2 of the stored (rounded) residual
+        Spinor<SF>::load(r + off, rs ? rs + so : nullptr, stride, (unsigned)i, vr);
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                acc = fma((double)vr[s][c].r, (double)vr[s][c].r, acc);
+                acc = fma((double)vr[s][c].i, (double)vr[s][c].i, acc);
+            }
+    }
+    block_reduce_store(acc, 0.0, partials + (size_t)rhs * gridDim.x + blockIdx.x);
+}
+
+// x (F64) += x_s (F32); x_s = 0
+__global__ __launch_bounds__(kBlock) void k_mx_relx(double2 *__restrict__ x,
+                                                    char *__restrict__ xs, size_t x_rhs_elems,
+                                                    size_t xs_rhs_bytes, int stride,
+                                                    const int *__restrict__ reliable)
+{
+    const int rhs = blockIdx.y;
+    if (!reliable[rhs])
+        return;
+    const int e = blockIdx.x * kBlock + threadIdx.x;
+    if (e >= 2 * stride)
+        return;
+    const int par = e / stride, i = e - par * stride;
+    double2 *xp = x + rhs * x_rhs_elems + (size_t)par * 12 * stride;
+    char *sp = xs + rhs * xs_rhs_bytes + (size_t)par * spinor_chunks(WD_F32) * 16 * stride;
+    cpx<double> vx[4][3];
+    cpx<float> vs[4][3], z[4][3];
+    Spinor<WD_F64>::load(xp, nullptr, stride, (unsigned)i, vx);
+    Spinor<WD_F32>::load(sp, nullptr, stride, (unsigned)i, vs);
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            vx[s][c].r += (double)vs[s][c].r;
+            vx[s][c].i += (double)vs[s][c].i;
+            z[s][c] = {0.f, 0.f};
+        }
+    Spinor<WD_F64>::store(xp, nullptr, stride, (unsigned)i, vx);
+    Spinor<WD_F32>::store(sp, nullptr, stride, (unsigned)i, z);
+}
+
+// r = bp - q64 (F64) -> r_s (SF); partial |r|^2.  With reliable == nullptr every rhs runs
+// (initialisation).
+template <int SF>
+__global__ __launch_bounds__(kBlock) void k_mx_relr(char *__restrict__ rsl, float *__restrict__ rss,
+                                                    const double2 *__restrict__ bp,
+                                                    const double2 *__restrict__ q64,
+                                                    size_t x_rhs_elems, size_t rhs_bytes,
+                                                    int stride, const int *__restrict__ reliable,
+                                                    double2 *__restrict__ partials)
+{
+    const int rhs = blockIdx.y;
+    if (reliable && !reliable[rhs])
+        return;
+    const int e = blockIdx.x * kBlock + threadIdx.x;
+    double acc = 0.0;
+    if (e < 2 * stride) {
+        const int par = e / stride, i = e - par * stride;
+        const size_t xo = rhs * x_rhs_elems + (size_t)par * 12 * stride;
+        cpx<double> vb[4][3], vq[4][3];
+        Spinor<WD_F64>::load(bp + xo, nullptr, stride, (unsigned)i, vb);
+        if (q64) {
+            Spinor<WD_F64>::load(q64 + xo, nullptr, stride, (unsigned)i, vq);
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    vb[s][c] = {vb[s][c].r - vq[s][c].r, vb[s][c].i - vq[s][c].i};
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                acc = fma(vb[s][c].r, vb[s][c].r, acc);
+                acc = fma(vb[s][c].i, vb[s][c].i, acc);
+            }
+        const size_t off = rhs * rhs_bytes + (size_t)par * spinor_chunks(SF) * 16 * stride;
+        const size_t so = ((size_t)rhs * 2 + par) * stride;
+        Spinor<SF>::store(rsl + off, rss ? rss + so : nullptr, stride, (unsigned)i, vb);
+    }
+    block_reduce_store(acc, 0.0, partials + (size_t)rhs * gridDim.x + blockIdx.x);
+}
+
+// ---- per-rhs scalar kernels: blockIdx.x = rhs, 1024 threads -------------------
+__global__ __launch_bounds__(1024) void k_mx_init(const double2 *__restrict__ partials, int count,
+                                                  MxState *__restrict__ st, int *__restrict__ active,
+                                                  int *__restrict__ reliable, double tol)
+{
+    const double2 s = sum_rhs_partials(partials, count);
+    if (threadIdx.x == 0) {
+        const int r = blockIdx.x;
+        MxState m = {};
+        m.rho = s.x;
+        m.b_norm = sqrt(s.x);
+        m.maxr = m.b_norm;
+        m.target = tol * m.b_norm;
+        m.rsd = m.b_norm;
+        m.true_rsd = m.b_norm;
+        m.beta = 0.0;
+        st[r] = m;
+        reliable[r] = 0;
+        // a zero right-hand side is solved by x = 0
+        if (!(m.b_norm > 0.0)) {
+            st[r].status = 1;
+            active[r] = 0;
+        } else {
+            active[r] = 1;
+        }
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_mx_alpha(const double2 *__restrict__ partials, int count,
+                                                   MxState *__restrict__ st,
+                                                   int *__restrict__ active)
+{
+    const int r = blockIdx.x;
+    if (!active[r])
+        return;
+    const double2 pq = sum_rhs_partials(partials, count);
+    if (threadIdx.x == 0) {
+        if (!isfinite(pq.x) || !(pq.x > 0.0)) {
+            st[r].status = 2;
+            active[r] = 0;
+            return;
+        }
+        st[r].alpha = st[r].rho / pq.x;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_mx_decide(const double2 *__restrict__ partials,
+                                                    int count, MxState *__restrict__ st,
+                                                    const int *__restrict__ active,
+                                                    int *__restrict__ reliable, int max_iter,
+                                                    double delta)
+{
+    const int r = blockIdx.x;
+    if (!active[r])
+        return;
+    const double2 s = sum_rhs_partials(partials, count);
+    if (threadIdx.x == 0) {
+        MxState &m = st[r];
+        const double rr = s.x, rsd = sqrt(rr);
+        m.iter += 1;
+        m.rsd = rsd;
+        if (rsd < delta * m.maxr || rsd < m.target || m.iter >= max_iter || !isfinite(rr)) {
+            reliable[r] = 1;
+        } else {
+            m.beta = rr / m.rho;
+            m.rho = rr;
+        }
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_mx_relfin(const double2 *__restrict__ partials,
+                                                    int count, MxState *__restrict__ st,
+                                                    int *__restrict__ active,
+                                                    int *__restrict__ reliable, int max_iter)
+{
+    const int r = blockIdx.x;
+    if (!reliable[r])
+        return;
+    const double2 s = sum_rhs_partials(partials, count);
+    if (threadIdx.x == 0) {
+        MxState &m = st[r];
+        const double rr = s.x, rsd = sqrt(rr);
+        m.true_rsd = rsd;
+        m.updates += 1;
+        reliable[r] = 0;
+        if (rsd < m.target) {
+            m.status = 1;
+            active[r] = 0;
+        } else if (m.iter >= max_iter || !isfinite(rr)) {
+            m.status = isfinite(rr) ? 3 : 2;
+            active[r] = 0;
+        } else {
+            m.beta = rr / m.rho;
+            m.rho = rr;
+            m.maxr = rsd;
+        }
+    }
+}
+
+__global__ void k_mx_alldone(const int *__restrict__ active, int nrhs, int *__restrict__ stop)
+{
+    if (threadIdx.x == 0) {
+        int any = 0;
+        for (int r = 0; r < nrhs; ++r)
+            any |= active[r];
+        *stop = any ? 0 : 1;
+    }
+}
+
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf()
+    {
+        if (p)
+            (void)hipFree(p);
+    }
+    int alloc(wd_ctx *c, size_t bytes)
+    {
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+            return wd_set_error(c, WD_ENOMEM, "mixed cg: allocation failed");
+        }
+        return wd_launch_zero(c, p, (bytes + 15) / 16 * 16);
+    }
+};
+
+struct FieldBuf {
+    wd_field *f = nullptr;
+    ~FieldBuf() { wd_field_destroy(f); }
+};
+
+} // namespace
+
+// ---- autotuning of the sloppy operator ------------------------------------------
+// Times HOP_M over `nrhs` systems for every rhs-per-thread candidate and keeps
+// the fastest per (nrhs, spinor format, link format).
+int wd_tune_sloppy(wd_ctx *ctx, const SloppyArgs &a0, int sprec, int gprec, int recon,
+                   int *best_R, double ms_out[3])
+{
+    const auto key = std::make_tuple(a0.nrhs, sprec, gprec * 100 + recon);
+    auto it = ctx->tuned_R.find(key);
+    if (it != ctx->tuned_R.end() && !ms_out) {
+        *best_R = it->second;
+        return WD_OK;
+    }
+    const int Rs[3] = {1, 2, 4};
+    double best = 1e30;
+    int bestR = 1;
+    hipEvent_t e0, e1;
+    WD_HIP_CHECK(ctx, hipEventCreate(&e0));
+    WD_HIP_CHECK(ctx, hipEventCreate(&e1));
+    int rc = WD_OK;
+    for (int k = 0; k < 3 && !rc; ++k) {
+        for (int w = 0; w < 2 && !rc; ++w)
+            rc = wd_launch_hop_sloppy(ctx, a0, sprec, gprec, recon, Rs[k], 0, -1, HOP_M);
+        if (rc)
+            break;
+        (void)hipEventRecord(e0, ctx->stream);
+        const int reps = 3;
+        for (int w = 0; w < reps && !rc; ++w)
+            rc = wd_launch_hop_sloppy(ctx, a0, sprec, gprec, recon, Rs[k], 0, -1, HOP_M);
+        (void)hipEventRecord(e1, ctx->stream);
+        (void)hipEventSynchronize(e1);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        const double t = ms / reps;
+        if (ms_out)
+            ms_out[k] = t;
+        if (t < best) {
+            best = t;
+            bestR = Rs[k];
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc)
+        return rc;
+    ctx->tuned_R[key] = bestR;
+    *best_R = bestR;
+    return WD_OK;
+}
+
+extern "C" void wd_mixed_params_default(wd_mixed_params *p)
+{
+    if (!p)
+        return;
+    p->max_iter = 100000;
+    p->tol = 1e-10;
+    p->delta = 1e-2;
+    p->spinor_prec = WD_F16;
+    p->rhs_per_thread = 0;
+    p->check_every = 16;
+    p->use_graph = 1;
+}
+
+extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd_gauge *U,
+                           const wd_gauge *Us, const double *masses, const wd_mixed_params *prm,
+                           wd_mixed_report *reports, double *seconds)
+{
+    if (!ctx)
+        return WD_EARG;
+    WD_ARG_CHECK(ctx, x && b && U && Us && masses && prm, "mixed cg: null argument");
+    WD_ARG_CHECK(ctx, x != b, "mixed cg: x and b must be distinct fields");
+    WD_ARG_CHECK(ctx, x->prec == WD_F64 && b->prec == WD_F64, "mixed cg: x and b must be F64");
+    WD_ARG_CHECK(ctx, x->nrhs == b->nrhs, "mixed cg: x and b nrhs differ");
+    WD_ARG_CHECK(ctx, U->prec == WD_F64 && U->recon == WD_RECON_18,
+                 "mixed cg: U must be F64 18-real (reliable updates)");
+    WD_ARG_CHECK(ctx, prm->spinor_prec == WD_F16 || prm->spinor_prec == WD_F32,
+                 "mixed cg: spinor_prec must be F16 or F32");
+    WD_ARG_CHECK(ctx, wd_sloppy_supported(prm->spinor_prec, Us->prec, Us->recon),
+                 "mixed cg: unsupported sloppy link format");
+    WD_ARG_CHECK(ctx, prm->rhs_per_thread == 0 || prm->rhs_per_thread == 1 ||
+                          prm->rhs_per_thread == 2 || prm->rhs_per_thread == 4,
+                 "mixed cg: rhs_per_thread must be 0, 1, 2 or 4");
+    WD_ARG_CHECK(ctx, prm->max_iter >= 1 && prm->check_every >= 1 && prm->tol > 0.0 &&
+                          prm->delta > 0.0 && prm->delta < 1.0,
+                 "mixed cg: bad iteration / tolerance parameters");
+    const auto t0 = std::chrono::steady_clock::now();
+    const int n = b->nrhs;
+    const int sp = prm->spinor_prec;
+    const int stride = (int)ctx->stride;
+    int rc;
+
+    // device work vectors
+    FieldBuf Mdb, t64, q64, rs, p, q, qq, xs;
+    if ((rc = wd_field_create(ctx, &Mdb.f, WD_F64, n)) || (rc = wd_field_create(ctx, &t64.f, WD_F64, n)) ||
+        (rc = wd_field_create(ctx, &q64.f, WD_F64, n)) || (rc = wd_field_create(ctx, &rs.f, sp, n)) ||
+        (rc = wd_field_create(ctx, &p.f, sp, n)) || (rc = wd_field_create(ctx, &q.f, sp, n)) ||
+        (rc = wd_field_create(ctx, &qq.f, sp, n)) || (rc = wd_field_create(ctx, &xs.f, WD_F32, n)))
+        return rc;
+    const int eg = (2 * stride + kBlock - 1) / kBlock; // elementwise workgroups per rhs
+    const int hop_nsb = 2 * (int)((ctx->Vh + kBlock - 1) / kBlock);
+    const int nsb_max = eg > hop_nsb ? eg : hop_nsb;
+    DevBuf partials, state, flags, diag_d, diag_f;
+    if ((rc = partials.alloc(ctx, sizeof(double2) * (size_t)n * nsb_max)) ||
+        (rc = state.alloc(ctx, sizeof(MxState) * n)) ||
+        (rc = flags.alloc(ctx, sizeof(int) * (2 * (size_t)n + 4))) ||
+        (rc = diag_d.alloc(ctx, sizeof(double) * n)) || (rc = diag_f.alloc(ctx, sizeof(float) * n)))
+        return rc;
+    MxState *st = (MxState *)state.p;
+    int *active = (int *)flags.p, *reliable = active + n, *stop = reliable + n;
+    double2 *part = (double2 *)partials.p;
+    {
+        std::vector<double> ad(n);
+        std::vector<float> af(n);
+        for (int r = 0; r < n; ++r) {
+            ad[r] = 4.0 + masses[r];
+            af[r] = (float)ad[r];
+        }
+        WD_HIP_CHECK(ctx, hipMemcpyAsync(diag_d.p, ad.data(), sizeof(double) * n,
+                                         hipMemcpyHostToDevice, ctx->stream));
+        WD_HIP_CHECK(ctx, hipMemcpyAsync(diag_f.p, af.data(), sizeof(float) * n,
+                                         hipMemcpyHostToDevice, ctx->stream));
+        WD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    const double2 *Ud = (const double2 *)U->data;
+    const size_t xe = x->rhs_elems;
+    const size_t srb = rs.f->rhs_elems * prec_bytes_complex(sp);
+    const size_t xsb = xs.f->rhs_elems * prec_bytes_complex(WD_F32);
+    F64Batch all = {xe, nullptr, (const double *)diag_d.p};
+    F64Batch rel = {xe, reliable, (const double *)diag_d.p};
+
+    // Mdb = M^dag b for every system; x = 0; r_s = Mdb; rho = |Mdb|^2
+    if ((rc = wd_launch_hop_f64_batch(ctx, (double2 *)Mdb.f->data, (const double2 *)b->data, Ud, 1,
+                                      -1, HOP_M, (const double2 *)b->data, nullptr, nullptr,
+                                      nullptr, all, n)))
+        return rc;
+    if ((rc = wd_launch_zero(ctx, x->data, x->bytes)))
+        return rc;
+    if (sp == WD_F16)
+        hipLaunchKernelGGL(k_mx_relr<WD_F16>, dim3(eg, n), dim3(kBlock), 0, ctx->stream,
+                           (char *)rs.f->data, rs.f->scale, (const double2 *)Mdb.f->data,
+                           (const double2 *)nullptr, xe, srb, stride, (const int *)nullptr, part);
+    else
+        hipLaunchKernelGGL(k_mx_relr<WD_F32>, dim3(eg, n), dim3(kBlock), 0, ctx->stream,
+                           (char *)rs.f->data, rs.f->scale, (const double2 *)Mdb.f->data,
+                           (const double2 *)nullptr, xe, srb, stride, (const int *)nullptr, part);
+    if ((rc = wd_check_launch(ctx, "k_mx_relr")))
+        return rc;
+    hipLaunchKernelGGL(k_mx_init, dim3(n), dim3(1024), 0, ctx->stream, part, eg, st, active,
+                       reliable, prm->tol);
+    hipLaunchKernelGGL(k_mx_alldone, dim3(1), dim3(64), 0, ctx->stream, active, n, stop);
+    if ((rc = wd_check_launch(ctx, "k_mx_init")))
+        return rc;
+
+    // sloppy operator arguments
+    SloppyArgs A = {};
+    A.rhs_bytes = srb;
+    A.U = (const char *)Us->data;
+    A.diag = (const float *)diag_f.p;
+    A.active = active;
+    A.partials = part;
+    A.stop = stop;
+    A.nrhs = n;
+    int R = prm->rhs_per_thread;
+    if (R == 0) {
+        SloppyArgs T = A;
+        T.out = (char *)qq.f->data;
+        T.out_scale = qq.f->scale;
+        T.in = T.same = (const char *)p.f->data;
+        T.in_scale = T.same_scale = p.f->scale;
+        if ((rc = wd_tune_sloppy(ctx, T, sp, Us->prec, Us->recon, &R, nullptr)))
+            return rc;
+    }
+    const int nsb = hop_nsb;
+
+    auto iteration = [&]() -> int {
+        int e;
+        if (sp == WD_F16)
+            hipLaunchKernelGGL(k_mx_xpay<WD_F16>, dim3(eg, n), dim3(kBlock), 0, ctx->stream,
+                               (char *)p.f->data, p.f->scale, (const char *)rs.f->data,
+                               (const float *)rs.f->scale, srb, stride, (const MxState *)st,
+                               (const int *)active);
+        else
+            hipLaunchKernelGGL(k_mx_xpay<WD_F32>, dim3(eg, n), dim3(kBlock), 0, ctx->stream,
+                               (char *)p.f->data, p.f->scale, (const char *)rs.f->data,
+                               (const float *)rs.f->scale, srb, stride, (const MxState *)st,
+                               (const int *)active);
+        if ((e = wd_check_launch(ctx, "k_mx_xpay")))
+            return e;
+        SloppyArgs m = A;
+        m.out = (char *)qq.f->data;
+        m.out_scale = qq.f->scale;
+        m.in = m.same = (const char *)p.f->data;
+        m.in_scale = m.same_scale = p.f->scale;
+        if ((e = wd_launch_hop_sloppy(ctx, m, sp, Us->prec, Us->recon, R, 0, -1, HOP_M)))
+            return e;
+        SloppyArgs md = A;
+        md.out = (char *)q.f->data;
+        md.out_scale = q.f->scale;
+        md.in = md.same = (const char *)qq.f->data;
+        md.in_scale = md.same_scale = qq.f->scale;
+        md.dotv = (const char *)p.f->data;
+        md.dotv_scale = p.f->scale;
+        if ((e = wd_launch_hop_sloppy(ctx, md, sp, Us->prec, Us->recon, R, 1, -1, HOP_M_DOT)))
+            return e;
+        hipLaunchKernelGGL(k_mx_alpha, dim3(n), dim3(1024), 0, ctx->stream,
+                           (const double2 *)part, nsb, st, active);
+        if (sp == WD_F16)
+            hipLaunchKernelGGL(k_mx_update<WD_F16>, dim3(eg, n), dim3(kBlock), 0, ctx->stream,
+                               (char *)xs.f->data, (char *)rs.f->data, rs.f->scale,
+                               (const char *)p.f->data, (const float *)p.f->scale,
+                               (const char *)q.f->data, (const float *)q.f->scale, srb, xsb,
+                               stride, (const MxState *)st, (const int *)active, part);
+        else
+            hipLaunchKernelGGL(k_mx_update<WD_F32>, dim3(eg, n), dim3(kBlock), 0, ctx->stream,
+                               (char *)xs.f->data, (char *)rs.f->data, rs.f->scale,
+                               (const char *)p.f->data, (const float *)p.f->scale,
+                               (const char *)q.f->data, (const float *)q.f->scale, srb, xsb,
+                               stride, (const MxState *)st, (const int *)active, part);
+        hipLaunchKernelGGL(k_mx_decide, dim3(n), dim3(1024), 0, ctx->stream,
+                           (const double2 *)part, eg, st, (const int *)active, reliable,
+                           prm->max_iter, prm->delta);
+        hipLaunchKernelGGL(k_mx_relx, dim3(eg, n), dim3(kBlock), 0, ctx->stream,
+                           (double2 *)x->data, (char *)xs.f->data, xe, xsb, stride,
+                           (const int *)reliable);
+        if ((e = wd_check_launch(ctx, "k_mx_relx")))
+            return e;
+        if ((e = wd_launch_hop_f64_batch(ctx, (double2 *)t64.f->data, (const double2 *)x->data,
+                                         Ud, 0, -1, HOP_M, (const double2 *)x->data, nullptr,
+                                         nullptr, stop, rel, n)) ||
+            (e = wd_launch_hop_f64_batch(ctx, (double2 *)q64.f->data,
+                                         (const double2 *)t64.f->data, Ud, 1, -1, HOP_M,
+                                         (const double2 *)t64.f->data, nullptr, nullptr, stop,
+                                         rel, n)))
+            return e;
+        if (sp == WD_F16)
+            hipLaunchKernelGGL(k_mx_relr<WD_F16>, dim3(eg, n), dim3(kBlock), 0, ctx->stream,
+                               (char *)rs.f->data, rs.f->scale, (const double2 *)Mdb.f->data,
+                               (const double2 *)q64.f->data, xe, srb, stride,
+                               (const int *)reliable, part);
+        else
+            hipLaunchKernelGGL(k_mx_relr<WD_F32>, dim3(eg, n), dim3(kBlock), 0, ctx->stream,
+                               (char *)rs.f->data, rs.f->scale, (const double2 *)Mdb.f->data,
+                               (const double2 *)q64.f->data, xe, srb, stride,
+                               (const int *)reliable, part);
+        hipLaunchKernelGGL(k_mx_relfin, dim3(n), dim3(1024), 0, ctx->stream,
+                           (const double2 *)part, eg, st, active, reliable, prm->max_iter);
+        hipLaunchKernelGGL(k_mx_alldone, dim3(1), dim3(64), 0, ctx->stream,
+                           (const int *)active, n, stop);
+        return wd_check_launch(ctx, "mixed cg iteration");
+    };
+
+    const int chunk = prm->check_every;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    if (prm->use_graph) {
+        WD_HIP_CHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        int e = 0;
+        for (int k = 0; k < chunk && !e; ++k)
+            e = iteration();
+        hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+        if (e)
+            return e;
+        WD_HIP_CHECK(ctx, ce);
+        ce = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+        if (ce != hipSuccess) {
+            (void)hipGraphDestroy(graph);
+            return wd_set_error(ctx, WD_EHIP,
+                                std::string("mixed cg: graph instantiate: ") + hipGetErrorString(ce));
+        }
+    }
+    int status = WD_OK;
+    long launched = 0;
+    const long limit = (long)prm->max_iter + chunk; // the device forces an exit at max_iter
+    while (true) {
+        int hstop = 0;
+        WD_HIP_CHECK(ctx, hipMemcpyAsync(ctx->hpinned, stop, sizeof(int), hipMemcpyDeviceToHost,
+                                         ctx->stream));
+        WD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        hstop = *(int *)ctx->hpinned;
+        if (hstop || launched > limit)
+            break;
+        if (gexec) {
+            hipError_t ge = hipGraphLaunch(gexec, ctx->stream);
+            if (ge != hipSuccess) {
+                status = wd_set_error(ctx, WD_EHIP,
+                                      std::string("mixed cg: graph launch: ") + hipGetErrorString(ge));
+                break;
+            }
+        } else {
+            for (int k = 0; k < chunk && !status; ++k)
+                status = iteration();
+            if (status)
+                break;
+        }
+        launched += chunk;
+    }
+    if (gexec)
+        (void)hipGraphExecDestroy(gexec);
+    if (graph)
+        (void)hipGraphDestroy(graph);
+    if (status)
+        return status;
+    std::vector<MxState> hs(n);
+    WD_HIP_CHECK(ctx, hipMemcpy(hs.data(), st, sizeof(MxState) * n, hipMemcpyDeviceToHost));
+    const auto t1 = std::chrono::steady_clock::now();
+    if (seconds)
+        *seconds = std::chrono::duration<double>(t1 - t0).count();
+    bool all_conv = true, breakdown = false;
+    for (int r = 0; r < n; ++r) {
+        if (reports) {
+            reports[r].iterations = hs[r].iter;
+            reports[r].reliable_updates = hs[r].updates;
+            reports[r].converged = hs[r].status == 1;
+            reports[r].true_rel_rsd = hs[r].b_norm > 0 ? hs[r].true_rsd / hs[r].b_norm : 0.0;
+            reports[r].rhs_norm = hs[r].b_norm;
+            reports[r].rhs_per_thread = R;
+        }
+        all_conv = all_conv && hs[r].status == 1;
+        breakdown = breakdown || hs[r].status == 2;
+    }
+    if (breakdown)
+        return wd_set_error(ctx, WD_EBREAKDOWN, "mixed cg: breakdown in at least one system");
+    return all_conv ? WD_OK : WD_ENOTCONV;
+}
+
+extern "C" int wd_tune_report(wd_ctx *ctx, const wd_field *f, const wd_gauge *Us, int *best_R,
+                              double ms[3])
+{
+    if (!ctx || !f || !Us || !best_R || !ms)
+        return WD_EARG;
+    WD_ARG_CHECK(ctx, wd_sloppy_supported(f->prec, Us->prec, Us->recon),
+                 "tune: unsupported precision combination");
+    wd_field *out = nullptr;
+    int rc = wd_field_create(ctx, &out, f->prec, f->nrhs);
+    if (rc)
+        return rc;
+    DevBuf diag;
+    if ((rc = diag.alloc(ctx, sizeof(float) * f->nrhs))) {
+        wd_field_destroy(out);
+        return rc;
+    }
+    SloppyArgs a = {};
+    a.out = (char *)out->data;
+    a.out_scale = out->scale;
+    a.in = a.same = (const char *)f->data;
+    a.in_scale = a.same_scale = f->scale;
+    a.rhs_bytes = f->rhs_elems * prec_bytes_complex(f->prec);
+    a.U = (const char *)Us->data;
+    a.diag = (const float *)diag.p;
+    a.nrhs = f->nrhs;
+    rc = wd_tune_sloppy(ctx, a, f->prec, Us->prec, Us->recon, best_R, ms);
+    wd_field_destroy(out);
+    return rc;
+}
+
+extern "C" int wd_apply_M_sloppy(wd_ctx *ctx, wd_field *out, const wd_field *in,
+                                 const wd_gauge *Us, const double *masses, int dagger,
+                                 int rhs_per_thread)
+{
+    if (!ctx)
+        return WD_EARG;
+    WD_ARG_CHECK(ctx, out && in && Us && masses, "apply_M_sloppy: null argument");
+    WD_ARG_CHECK(ctx, out != in, "apply_M_sloppy: out and in must be distinct fields");
+    WD_ARG_CHECK(ctx, out->prec == in->prec && out->nrhs == in->nrhs,
+                 "apply_M_sloppy: out and in must have the same format and nrhs");
+    WD_ARG_CHECK(ctx, wd_sloppy_supported(in->prec, Us->prec, Us->recon),
+                 "apply_M_sloppy: unsupported precision combination");
+    const int n = in->nrhs;
+    DevBuf diag;
+    int rc = diag.alloc(ctx, sizeof(float) * n);
+    if (rc)
+        return rc;
+    std::vector<float> af(n);
+    for (int r = 0; r < n; ++r)
+        af[r] = (float)(4.0 + masses[r]);
+    WD_HIP_CHECK(ctx, hipMemcpyAsync(diag.p, af.data(), sizeof(float) * n, hipMemcpyHostToDevice,
+                                     ctx->stream));
+    SloppyArgs a = {};
+    a.out = (char *)out->data;
+    a.out_scale = out->scale;
+    a.in = a.same = (const char *)in->data;
+    a.in_scale = a.same_scale = in->scale;
+    a.rhs_bytes = in->rhs_elems * prec_bytes_complex(in->prec);
+    a.U = (const char *)Us->data;
+    a.diag = (const float *)diag.p;
+    a.nrhs = n;
+    rc = wd_launch_hop_sloppy(ctx, a, in->prec, Us->prec, Us->recon, rhs_per_thread, dagger, -1,
+                              HOP_M);
+    if (rc)
+        return rc;
+    WD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return WD_OK;
+}

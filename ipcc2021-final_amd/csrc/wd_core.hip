@@ -52,137 +52,6 @@ extern "C" const char *wd_last_error(const wd_ctx *ctx)
     return ctx ? ctx->last_error.c_str() : "no context";
 }
 
-// ---- layout conversion kernels ---------------------------------------------------
-namespace {
-
-// ref spinor (complex<double>[V*12], eo*12 + k) -> device SoA of precision T
-__global__ __launch_bounds__(kBlock) void k_spinor_from_ref_f64(double2 *__restrict__ dst,
-                                                                const double2 *__restrict__ src,
-                                                                int Vh, int stride)
-{
-    const int e = blockIdx.x * kBlock + threadIdx.x; // eo site index in [0, 2*Vh)
-    if (e >= 2 * Vh)
-        return;
-    const int p = e >= Vh, i = e - p * Vh;
-    for (int k = 0; k < 12; ++k)
-        dst[(size_t)(p * 12 + k) * stride + i] = src[(size_t)e * 12 + k];
-}
-
-__global__ __launch_bounds__(kBlock) void k_spinor_to_ref_f64(double2 *__restrict__ dst,
-                                                              const double2 *__restrict__ src,
-                                                              int Vh, int stride)
-{
-    const int e = blockIdx.x * kBlock + threadIdx.x;
-    if (e >= 2 * Vh)
-        return;
-    const int p = e >= Vh, i = e - p * Vh;
-    for (int k = 0; k < 12; ++k)
-        dst[(size_t)e * 12 + k] = src[(size_t)(p * 12 + k) * stride + i];
-}
-
-__global__ __launch_bounds__(kBlock) void k_spinor_from_ref_f32(float2 *__restrict__ dst,
-                                                                const double2 *__restrict__ src,
-                                                                int Vh, int stride)
-{
-    const int e = blockIdx.x * kBlock + threadIdx.x;
-    if (e >= 2 * Vh)
-        return;
-    const int p = e >= Vh, i = e - p * Vh;
-    for (int k = 0; k < 12; ++k) {
-        const double2 v = src[(size_t)e * 12 + k];
-        dst[(size_t)(p * 12 + k) * stride + i] = make_float2((float)v.x, (float)v.y);
-    }
-}
-
-__global__ __launch_bounds__(kBlock) void k_spinor_to_ref_f32(double2 *__restrict__ dst,
-                                                              const float2 *__restrict__ src,
-                                                              int Vh, int stride)
-{
-    const int e = blockIdx.x * kBlock + threadIdx.x;
-    if (e >= 2 * Vh)
-        return;
-    const int p = e >= Vh, i = e - p * Vh;
-    for (int k = 0; k < 12; ++k) {
-        const float2 v = src[(size_t)(p * 12 + k) * stride + i];
-        dst[(size_t)e * 12 + k] = make_double2(v.x, v.y);
-    }
-}
-
-// F16 storage: per-site scale s = max |component|, values stored as v / s.
-__global__ __launch_bounds__(kBlock) void k_spinor_from_ref_f16(__half2 *__restrict__ dst,
-                                                                float *__restrict__ scale,
-                                                                const double2 *__restrict__ src,
-                                                                int Vh, int stride)
-{
-    const int e = blockIdx.x * kBlock + threadIdx.x;
-    if (e >= 2 * Vh)
-        return;
-    const int p = e >= Vh, i = e - p * Vh;
-    double m = 0.0;
-    for (int k = 0; k < 12; ++k) {
-        const double2 v = src[(size_t)e * 12 + k];
-        m = fmax(m, fmax(fabs(v.x), fabs(v.y)));
-    }
-    const float s = (float)m;
-    const float inv = s > 0.f ? 1.0f / s : 0.f;
-    scale[(size_t)p * stride + i] = s;
-    for (int k = 0; k < 12; ++k) {
-        const double2 v = src[(size_t)e * 12 + k];
-        dst[(size_t)(p * 12 + k) * stride + i] =
-            __floats2half2_rn((float)v.x * inv, (float)v.y * inv);
-    }
-}
-
-__global__ __launch_bounds__(kBlock) void k_spinor_to_ref_f16(double2 *__restrict__ dst,
-                                                              const __half2 *__restrict__ src,
-                                                              const float *__restrict__ scale,
-                                                              int Vh, int stride)
-{
-    const int e = blockIdx.x * kBlock + threadIdx.x;
-    if (e >= 2 * Vh)
-        return;
-    const int p = e >= Vh, i = e - p * Vh;
-    const float s = scale[(size_t)p * stride + i];
-    for (int k = 0; k < 12; ++k) {
-        const float2 v = __half22float2(src[(size_t)(p * 12 + k) * stride + i]);
-        dst[(size_t)e * 12 + k] = make_double2((double)v.x * s, (double)v.y * s);
-    }
-}
-
-// ref gauge A[mu] (complex<double>[V*9]) -> device [mu][parity][nrow*3][stride]
-template <typename T2>
-__device__ __forceinline__ T2 cvt2(double2 v);
-template <>
-__device__ __forceinline__ double2 cvt2<double2>(double2 v)
-{
-    return v;
-}
-template <>
-__device__ __forceinline__ float2 cvt2<float2>(double2 v)
-{
-    return make_float2((float)v.x, (float)v.y);
-}
-template <>
-__device__ __forceinline__ __half2 cvt2<__half2>(double2 v)
-{
-    return __floats2half2_rn((float)v.x, (float)v.y);
-}
-
-template <typename T2>
-__global__ __launch_bounds__(kBlock) void k_gauge_from_ref(T2 *__restrict__ dst,
-                                                           const double2 *__restrict__ src,
-                                                           int mu, int nrow, int Vh, int stride)
-{
-    const int e = blockIdx.x * kBlock + threadIdx.x;
-    if (e >= 2 * Vh)
-        return;
-    const int p = e >= Vh, i = e - p * Vh;
-    T2 *base = dst + (size_t)((mu * 2 + p) * nrow * 3) * stride;
-    for (int k = 0; k < nrow * 3; ++k)
-        base[(size_t)k * stride + i] = cvt2<T2>(src[(size_t)e * 9 + k]);
-}
-
-} // namespace
 
 int wd_ensure_staging(wd_ctx *ctx, size_t bytes)
 {
@@ -300,6 +169,8 @@ extern "C" int wd_gauge_create(wd_ctx *ctx, wd_gauge **U, int prec, int recon)
     *U = nullptr;
     WD_ARG_CHECK(ctx, prec == WD_F64 || prec == WD_F32 || prec == WD_F16, "gauge: bad precision");
     WD_ARG_CHECK(ctx, recon == WD_RECON_18 || recon == WD_RECON_12, "gauge: bad recon");
+    WD_ARG_CHECK(ctx, !(prec == WD_F16 && recon == WD_RECON_18),
+                 "gauge: F16 links are stored 12-real only");
     wd_gauge *g = new (std::nothrow) wd_gauge();
     if (!g)
         return WD_ENOMEM;
@@ -339,24 +210,11 @@ extern "C" int wd_gauge_upload_ref(wd_ctx *ctx, wd_gauge *U, const double *const
     int rc = wd_ensure_staging(ctx, bytes);
     if (rc)
         return rc;
-    const int nrow = U->recon == WD_RECON_18 ? 3 : 2;
-    const int grid = (int)((2 * ctx->Vh + kBlock - 1) / kBlock);
     for (int mu = 0; mu < 4; ++mu) {
         WD_ARG_CHECK(ctx, A[mu] != nullptr, "gauge upload: null direction array");
         WD_HIP_CHECK(ctx, hipMemcpyAsync(ctx->staging, A[mu], bytes, hipMemcpyHostToDevice,
                                          ctx->stream));
-        const double2 *src = (const double2 *)ctx->staging;
-        if (U->prec == WD_F64)
-            hipLaunchKernelGGL(k_gauge_from_ref<double2>, dim3(grid), dim3(kBlock), 0, ctx->stream,
-                               (double2 *)U->data, src, mu, nrow, (int)ctx->Vh, (int)ctx->stride);
-        else if (U->prec == WD_F32)
-            hipLaunchKernelGGL(k_gauge_from_ref<float2>, dim3(grid), dim3(kBlock), 0, ctx->stream,
-                               (float2 *)U->data, src, mu, nrow, (int)ctx->Vh, (int)ctx->stride);
-        else
-            hipLaunchKernelGGL(k_gauge_from_ref<__half2>, dim3(grid), dim3(kBlock), 0,
-                               ctx->stream, (__half2 *)U->data, src, mu, nrow, (int)ctx->Vh,
-                               (int)ctx->stride);
-        rc = wd_check_launch(ctx, "k_gauge_from_ref");
+        rc = wd_launch_gauge_from_ref(ctx, U, mu, (const double2 *)ctx->staging);
         if (rc)
             return rc;
     }
@@ -427,16 +285,6 @@ extern "C" int wd_field_zero(wd_ctx *ctx, wd_field *f)
     return WD_OK;
 }
 
-static void *rhs_ptr(const wd_field *f, int rhs)
-{
-    return (char *)f->data + (size_t)rhs * f->rhs_elems * prec_bytes_complex(f->prec);
-}
-
-static float *rhs_scale(const wd_field *f, int rhs)
-{
-    return f->scale ? f->scale + (size_t)rhs * 2 * f->ctx->stride : nullptr;
-}
-
 extern "C" int wd_field_upload_ref(wd_ctx *ctx, wd_field *f, const double *A, int rhs)
 {
     if (!ctx || !f || !A)
@@ -448,19 +296,7 @@ extern "C" int wd_field_upload_ref(wd_ctx *ctx, wd_field *f, const double *A, in
         return rc;
     WD_HIP_CHECK(ctx,
                  hipMemcpyAsync(ctx->staging, A, bytes, hipMemcpyHostToDevice, ctx->stream));
-    const int grid = (int)((2 * ctx->Vh + kBlock - 1) / kBlock);
-    const double2 *src = (const double2 *)ctx->staging;
-    const int Vh = (int)ctx->Vh, st = (int)ctx->stride;
-    if (f->prec == WD_F64)
-        hipLaunchKernelGGL(k_spinor_from_ref_f64, dim3(grid), dim3(kBlock), 0, ctx->stream,
-                           (double2 *)rhs_ptr(f, rhs), src, Vh, st);
-    else if (f->prec == WD_F32)
-        hipLaunchKernelGGL(k_spinor_from_ref_f32, dim3(grid), dim3(kBlock), 0, ctx->stream,
-                           (float2 *)rhs_ptr(f, rhs), src, Vh, st);
-    else
-        hipLaunchKernelGGL(k_spinor_from_ref_f16, dim3(grid), dim3(kBlock), 0, ctx->stream,
-                           (__half2 *)rhs_ptr(f, rhs), rhs_scale(f, rhs), src, Vh, st);
-    rc = wd_check_launch(ctx, "k_spinor_from_ref");
+    rc = wd_launch_spinor_from_ref(ctx, f, rhs, (const double2 *)ctx->staging);
     if (rc)
         return rc;
     WD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
@@ -476,19 +312,7 @@ extern "C" int wd_field_download_ref(wd_ctx *ctx, const wd_field *f, double *A, 
     int rc = wd_ensure_staging(ctx, bytes);
     if (rc)
         return rc;
-    const int grid = (int)((2 * ctx->Vh + kBlock - 1) / kBlock);
-    double2 *dst = (double2 *)ctx->staging;
-    const int Vh = (int)ctx->Vh, st = (int)ctx->stride;
-    if (f->prec == WD_F64)
-        hipLaunchKernelGGL(k_spinor_to_ref_f64, dim3(grid), dim3(kBlock), 0, ctx->stream, dst,
-                           (const double2 *)rhs_ptr(f, rhs), Vh, st);
-    else if (f->prec == WD_F32)
-        hipLaunchKernelGGL(k_spinor_to_ref_f32, dim3(grid), dim3(kBlock), 0, ctx->stream, dst,
-                           (const float2 *)rhs_ptr(f, rhs), Vh, st);
-    else
-        hipLaunchKernelGGL(k_spinor_to_ref_f16, dim3(grid), dim3(kBlock), 0, ctx->stream, dst,
-                           (const __half2 *)rhs_ptr(f, rhs), rhs_scale(f, rhs), Vh, st);
-    rc = wd_check_launch(ctx, "k_spinor_to_ref");
+    rc = wd_launch_spinor_to_ref(ctx, f, rhs, (double2 *)ctx->staging);
     if (rc)
         return rc;
     WD_HIP_CHECK(ctx, hipMemcpyAsync(A, ctx->staging, bytes, hipMemcpyDeviceToHost, ctx->stream));
@@ -628,127 +452,6 @@ extern "C" int wd_dot(wd_ctx *ctx, const wd_field *a, const wd_field *b, double 
     WD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     out[0] = ((double2 *)ctx->hpinned)->x;
     out[1] = ((double2 *)ctx->hpinned)->y;
-    return WD_OK;
-}
-
-// ---- precision conversion -----------------------------------------------------------------
-namespace {
-
-// F64 SoA -> F32 SoA, elementwise over one rhs (24 * stride complex)
-__global__ __launch_bounds__(kBlock) void k_f64_to_f32(float2 *__restrict__ dst,
-                                                       const double2 *__restrict__ src, size_t n)
-{
-    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * kBlock) {
-        const double2 v = src[i];
-        dst[i] = make_float2((float)v.x, (float)v.y);
-    }
-}
-
-__global__ __launch_bounds__(kBlock) void k_f32_to_f64(double2 *__restrict__ dst,
-                                                       const float2 *__restrict__ src, size_t n)
-{
-    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * kBlock) {
-        const float2 v = src[i];
-        dst[i] = make_double2(v.x, v.y);
-    }
-}
-
-// F64 SoA -> F16 SoA + per-site scale; one thread per (parity, site)
-__global__ __launch_bounds__(kBlock) void k_f64_to_f16(__half2 *__restrict__ dst,
-                                                       float *__restrict__ scale,
-                                                       const double2 *__restrict__ src, int stride)
-{
-    const int e = blockIdx.x * kBlock + threadIdx.x; // [0, 2*stride)
-    if (e >= 2 * stride)
-        return;
-    const int p = e / stride, i = e - p * stride;
-    const double2 *s = src + (size_t)p * 12 * stride + i;
-    double m = 0.0;
-    for (int k = 0; k < 12; ++k) {
-        const double2 v = s[(size_t)k * stride];
-        m = fmax(m, fmax(fabs(v.x), fabs(v.y)));
-    }
-    const float sc = (float)m;
-    const float inv = sc > 0.f ? 1.0f / sc : 0.f;
-    scale[e] = sc;
-    __half2 *d = dst + (size_t)p * 12 * stride + i;
-    for (int k = 0; k < 12; ++k) {
-        const double2 v = s[(size_t)k * stride];
-        d[(size_t)k * stride] = __floats2half2_rn((float)v.x * inv, (float)v.y * inv);
-    }
-}
-
-__global__ __launch_bounds__(kBlock) void k_f16_to_f64(double2 *__restrict__ dst,
-                                                       const __half2 *__restrict__ src,
-                                                       const float *__restrict__ scale,
-                                                       int stride)
-{
-    const int e = blockIdx.x * kBlock + threadIdx.x;
-    if (e >= 2 * stride)
-        return;
-    const int p = e / stride, i = e - p * stride;
-    const float sc = scale[e];
-    const __half2 *s = src + (size_t)p * 12 * stride + i;
-    double2 *d = dst + (size_t)p * 12 * stride + i;
-    for (int k = 0; k < 12; ++k) {
-        const float2 v = __half22float2(s[(size_t)k * stride]);
-        d[(size_t)k * stride] = make_double2((double)v.x * sc, (double)v.y * sc);
-    }
-}
-
-} // namespace
-
-// dst[rhs] = src[rhs] with precision conversion; F64 <-> F32 and F64 <-> F16,
-// or a plain copy for equal precisions.
-int wd_convert_rhs(wd_ctx *ctx, wd_field *dst, int rd, const wd_field *src, int rs)
-{
-    const size_t n = src->rhs_elems;
-    const int grid_e = wd_elem_grid(n);
-    const int grid_s = (int)((2 * ctx->stride + kBlock - 1) / kBlock);
-    void *d = rhs_ptr(dst, rd);
-    const void *s = rhs_ptr(src, rs);
-    if (dst->prec == src->prec) {
-        WD_HIP_CHECK(ctx, hipMemcpyAsync(d, s, n * prec_bytes_complex(src->prec),
-                                         hipMemcpyDeviceToDevice, ctx->stream));
-        if (dst->scale)
-            WD_HIP_CHECK(ctx, hipMemcpyAsync(rhs_scale(dst, rd), rhs_scale(src, rs),
-                                             sizeof(float) * 2 * ctx->stride,
-                                             hipMemcpyDeviceToDevice, ctx->stream));
-        return WD_OK;
-    }
-    if (src->prec == WD_F64 && dst->prec == WD_F32)
-        hipLaunchKernelGGL(k_f64_to_f32, dim3(grid_e), dim3(kBlock), 0, ctx->stream, (float2 *)d,
-                           (const double2 *)s, n);
-    else if (src->prec == WD_F32 && dst->prec == WD_F64)
-        hipLaunchKernelGGL(k_f32_to_f64, dim3(grid_e), dim3(kBlock), 0, ctx->stream, (double2 *)d,
-                           (const float2 *)s, n);
-    else if (src->prec == WD_F64 && dst->prec == WD_F16)
-        hipLaunchKernelGGL(k_f64_to_f16, dim3(grid_s), dim3(kBlock), 0, ctx->stream,
-                           (__half2 *)d, rhs_scale(dst, rd), (const double2 *)s,
-                           (int)ctx->stride);
-    else if (src->prec == WD_F16 && dst->prec == WD_F64)
-        hipLaunchKernelGGL(k_f16_to_f64, dim3(grid_s), dim3(kBlock), 0, ctx->stream,
-                           (double2 *)d, (const __half2 *)s, rhs_scale(src, rs),
-                           (int)ctx->stride);
-    else
-        return wd_set_error(ctx, WD_EARG, "copy: F32 <-> F16 conversion is not provided");
-    return wd_check_launch(ctx, "convert");
-}
-
-extern "C" int wd_field_copy(wd_ctx *ctx, wd_field *dst, const wd_field *src)
-{
-    if (!ctx || !dst || !src)
-        return WD_EARG;
-    WD_ARG_CHECK(ctx, dst != src, "copy: dst and src must be distinct");
-    WD_ARG_CHECK(ctx, dst->nrhs == src->nrhs, "copy: nrhs mismatch");
-    for (int r = 0; r < src->nrhs; ++r) {
-        int rc = wd_convert_rhs(ctx, dst, r, src, r);
-        if (rc)
-            return rc;
-    }
-    WD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     return WD_OK;
 }
 

@@ -1,0 +1,231 @@
+// wd_fields.h -- device access to every spinor and link storage format.
+//
+// Spinor fields, per right-hand side and parity, 16-byte chunks, site fastest:
+//   F64  double2 [12][stride]   chunk = 1 complex
+//   F32  float4  [ 6][stride]   chunk = 2 complex (components 2j, 2j+1)
+//   F16  uint4   [ 3][stride]   chunk = 4 complex as half2, values v / s with a
+//        float scale s [stride] = max |real component| of the site
+// Gauge fields, per direction and parity:
+//   F64 R18 double2 [9][stride]   F64 R12 double2 [6][stride]
+//   F32 R18 float2  [9][stride]   F32 R12 float4  [3][stride]
+//   F16 R12 uint2   [3][stride]   (chunk = 2 complex as half2; |U_ab| <= 1
+//        for unitary links, so no scale)
+// 12-real links hold rows 0 and 1; row 2 = conj(row0 x row1) is rebuilt in
+// registers (exact for SU(3); the 12-real storage of refs/utils_nersc_lat.cpp:60-68).
+#pragma once
+
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include "wd_dirac.h"
+#include "wd_internal.h"
+
+namespace wdf {
+
+typedef double d2v __attribute__((ext_vector_type(2)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef unsigned u4v __attribute__((ext_vector_type(4)));
+typedef unsigned u2v __attribute__((ext_vector_type(2)));
+
+// ---- half helpers ------------------------------------------------------------
+__device__ __forceinline__ cpx<float> h2f(unsigned w, float s)
+{
+    __half2 h = *reinterpret_cast<__half2 *>(&w);
+    const float2 f = __half22float2(h);
+    return {f.x * s, f.y * s};
+}
+
+__device__ __forceinline__ unsigned f2h(float re, float im)
+{
+    __half2 h = __floats2half2_rn(re, im);
+    return *reinterpret_cast<unsigned *>(&h);
+}
+
+// ---- spinor sites ---------------------------------------------------------------
+// base points at the parity block of one rhs; scale at that parity's scales (F16).
+template <int FMT>
+struct Spinor;
+
+template <>
+struct Spinor<WD_F64> {
+    using T = double;
+    template <typename R>
+    __device__ static __forceinline__ void load(const void *base, const float *, int stride,
+                                                unsigned i, cpx<R> (&v)[4][3])
+    {
+        const double2 *b = static_cast<const double2 *>(base);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            const d2v c = *reinterpret_cast<const d2v *>(b + (size_t)k * (unsigned)stride + i);
+            v[k / 3][k % 3] = {(R)c.x, (R)c.y};
+        }
+    }
+    template <typename R>
+    __device__ static __forceinline__ void store(void *base, float *, int stride, unsigned i,
+                                                 const cpx<R> (&v)[4][3])
+    {
+        double2 *b = static_cast<double2 *>(base);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            d2v c;
+            c.x = (double)v[k / 3][k % 3].r;
+            c.y = (double)v[k / 3][k % 3].i;
+            *reinterpret_cast<d2v *>(b + (size_t)k * (unsigned)stride + i) = c;
+        }
+    }
+};
+
+template <>
+struct Spinor<WD_F32> {
+    using T = float;
+    template <typename R>
+    __device__ static __forceinline__ void load(const void *base, const float *, int stride,
+                                                unsigned i, cpx<R> (&v)[4][3])
+    {
+        const float4 *b = static_cast<const float4 *>(base);
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            const f4v c = *reinterpret_cast<const f4v *>(b + (size_t)j * (unsigned)stride + i);
+            v[(2 * j) / 3][(2 * j) % 3] = {(R)c.x, (R)c.y};
+            v[(2 * j + 1) / 3][(2 * j + 1) % 3] = {(R)c.z, (R)c.w};
+        }
+    }
+    template <typename R>
+    __device__ static __forceinline__ void store(void *base, float *, int stride, unsigned i,
+                                                 const cpx<R> (&v)[4][3])
+    {
+        float4 *b = static_cast<float4 *>(base);
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            f4v c;
+            c.x = (float)v[(2 * j) / 3][(2 * j) % 3].r;
+            c.y = (float)v[(2 * j) / 3][(2 * j) % 3].i;
+            c.z = (float)v[(2 * j + 1) / 3][(2 * j + 1) % 3].r;
+            c.w = (float)v[(2 * j + 1) / 3][(2 * j + 1) % 3].i;
+            *reinterpret_cast<f4v *>(b + (size_t)j * (unsigned)stride + i) = c;
+        }
+    }
+};
+
+template <>
+struct Spinor<WD_F16> {
+    using T = float;
+    template <typename R>
+    __device__ static __forceinline__ void load(const void *base, const float *scale, int stride,
+                                                unsigned i, cpx<R> (&v)[4][3])
+    {
+        const uint4 *b = static_cast<const uint4 *>(base);
+        const float s = scale[i];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const u4v c = *reinterpret_cast<const u4v *>(b + (size_t)j * (unsigned)stride + i);
+            const unsigned w[4] = {c.x, c.y, c.z, c.w};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const cpx<float> f = h2f(w[q], s);
+                v[(4 * j + q) / 3][(4 * j + q) % 3] = {(R)f.r, (R)f.i};
+            }
+        }
+    }
+    template <typename R>
+    __device__ static __forceinline__ void store(void *base, float *scale, int stride, unsigned i,
+                                                 const cpx<R> (&v)[4][3])
+    {
+        uint4 *b = static_cast<uint4 *>(base);
+        float m = 0.f;
+#pragma unroll
+        for (int k = 0; k < 12; ++k)
+            m = fmaxf(m, fmaxf(fabsf((float)v[k / 3][k % 3].r), fabsf((float)v[k / 3][k % 3].i)));
+        const float inv = m > 0.f ? 1.0f / m : 0.f;
+        scale[i] = m;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            unsigned w[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const cpx<R> z = v[(4 * j + q) / 3][(4 * j + q) % 3];
+                w[q] = f2h((float)z.r * inv, (float)z.i * inv);
+            }
+            u4v c;
+            c.x = w[0];
+            c.y = w[1];
+            c.z = w[2];
+            c.w = w[3];
+            *reinterpret_cast<u4v *>(b + (size_t)j * (unsigned)stride + i) = c;
+        }
+    }
+};
+
+// chunks per parity block and bytes per chunk of a spinor format
+__host__ __device__ constexpr int spinor_chunks(int fmt) { return fmt == WD_F64 ? 12 : fmt == WD_F32 ? 6 : 3; }
+
+// ---- links ----------------------------------------------------------------------
+// GF: 0 = F64 R18, 1 = F64 R12, 2 = F32 R18, 3 = F32 R12, 4 = F16 R12
+enum GaugeFmt { G_F64_18 = 0, G_F64_12 = 1, G_F32_18 = 2, G_F32_12 = 3, G_F16_12 = 4 };
+
+__host__ __device__ constexpr int gauge_fmt(int prec, int recon)
+{
+    return prec == WD_F64 ? (recon == WD_RECON_18 ? G_F64_18 : G_F64_12)
+         : prec == WD_F32 ? (recon == WD_RECON_18 ? G_F32_18 : G_F32_12)
+                          : G_F16_12;
+}
+
+// chunks per (mu, parity) block
+__host__ __device__ constexpr int gauge_chunks(int gf)
+{
+    return gf == G_F64_18 ? 9 : gf == G_F64_12 ? 6 : gf == G_F32_18 ? 9 : 3;
+}
+__host__ __device__ constexpr int gauge_chunk_bytes(int gf)
+{
+    return gf == G_F64_18 || gf == G_F64_12 ? 16 : gf == G_F32_18 ? 8 : gf == G_F32_12 ? 16 : 8;
+}
+
+template <int GF, typename R>
+__device__ __forceinline__ void load_link(const void *base, int stride, unsigned j,
+                                          cpx<R> (&U)[3][3])
+{
+    if constexpr (GF == G_F64_18 || GF == G_F64_12) {
+        const double2 *b = static_cast<const double2 *>(base);
+        constexpr int n = GF == G_F64_18 ? 9 : 6;
+#pragma unroll
+        for (int k = 0; k < n; ++k) {
+            const d2v c = __builtin_nontemporal_load(
+                reinterpret_cast<const d2v *>(b + (size_t)k * (unsigned)stride + j));
+            U[k / 3][k % 3] = {(R)c.x, (R)c.y};
+        }
+        if constexpr (GF == G_F64_12)
+            reconstruct_row2(U);
+    } else if constexpr (GF == G_F32_18) {
+        const float2 *b = static_cast<const float2 *>(base);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            const f2v c = __builtin_nontemporal_load(
+                reinterpret_cast<const f2v *>(b + (size_t)k * (unsigned)stride + j));
+            U[k / 3][k % 3] = {(R)c.x, (R)c.y};
+        }
+    } else if constexpr (GF == G_F32_12) {
+        const float4 *b = static_cast<const float4 *>(base);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const f4v c = __builtin_nontemporal_load(
+                reinterpret_cast<const f4v *>(b + (size_t)q * (unsigned)stride + j));
+            U[(2 * q) / 3][(2 * q) % 3] = {(R)c.x, (R)c.y};
+            U[(2 * q + 1) / 3][(2 * q + 1) % 3] = {(R)c.z, (R)c.w};
+        }
+        reconstruct_row2(U);
+    } else {
+        const uint2 *b = static_cast<const uint2 *>(base);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const u2v c = __builtin_nontemporal_load(
+                reinterpret_cast<const u2v *>(b + (size_t)q * (unsigned)stride + j));
+            const cpx<float> a = h2f(c.x, 1.0f), d = h2f(c.y, 1.0f);
+            U[(2 * q) / 3][(2 * q) % 3] = {(R)a.r, (R)a.i};
+            U[(2 * q + 1) / 3][(2 * q + 1) % 3] = {(R)d.r, (R)d.i};
+        }
+        reconstruct_row2(U);
+    }
+}
+
+} // namespace wdf

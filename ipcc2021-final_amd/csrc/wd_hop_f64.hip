@@ -108,12 +108,30 @@ __global__ __launch_bounds__(kBlock) void k_hop_f64(double2 *__restrict__ out,
                                                     const double2 *__restrict__ same,
                                                     const double2 *__restrict__ dotv,
                                                     double2 *__restrict__ partials, int remap,
-                                                    const int *__restrict__ stop)
+                                                    const int *__restrict__ stop, F64Batch b)
 {
     // solver loops launch ahead of their convergence check; a set stop flag
     // turns the remaining launches into no-ops
     if (stop && *stop)
         return;
+    // right-hand side blockIdx.y of a batch: masked systems are skipped,
+    // each may carry its own diagonal coefficient a_r = 4 + m_r
+    {
+        const int rhs = blockIdx.y;
+        if (b.mask && !b.mask[rhs])
+            return;
+        if (b.a_rhs)
+            a = b.a_rhs[rhs];
+        const size_t ro = (size_t)rhs * b.rhs_elems;
+        out += ro;
+        in += ro;
+        if (same)
+            same += ro;
+        if (dotv)
+            dotv += ro;
+        if (partials)
+            partials += (size_t)rhs * gridDim.x;
+    }
     const int lb = logical_block(remap);
     const int bpp = gridDim.x / nparity; // workgroups per parity
     const int p = (nparity == 2) ? lb / bpp : cb_first;
@@ -232,28 +250,29 @@ __global__ __launch_bounds__(kBlock) void k_hop_f64(double2 *__restrict__ out,
 template <bool DAG>
 int launch_mode(wd_ctx *ctx, double2 *out, const double2 *in, const double2 *U, int cb, int mode,
                 double a, const double2 *same, const double2 *dotv, double2 *partials,
-                const int *stop)
+                const int *stop, const F64Batch &b, int nrhs)
 {
     const Geom &g = ctx->geom;
     const int nparity = (cb < 0) ? 2 : 1;
     const int bpp = (g.Vh + kBlock - 1) / kBlock;
     const int grid = bpp * nparity;
     const int cb_first = (cb < 0) ? 0 : cb;
-    if (mode == HOP_M_DOT && grid > ctx->max_partials)
+    if (mode == HOP_M_DOT && partials == ctx->partials && (long)grid * nrhs > ctx->max_partials)
         return wd_set_error(ctx, WD_EARG, "hop: too many workgroups for the partials buffer");
+    const dim3 gd(grid, nrhs);
     switch (mode) {
     case HOP_ONLY:
-        hipLaunchKernelGGL((k_hop_f64<DAG, HOP_ONLY>), dim3(grid), dim3(kBlock), 0, ctx->stream, out,
-                           in, U, g, cb_first, nparity, a, same, dotv, partials, ctx->xcd_remap, stop);
+        hipLaunchKernelGGL((k_hop_f64<DAG, HOP_ONLY>), gd, dim3(kBlock), 0, ctx->stream, out,
+                           in, U, g, cb_first, nparity, a, same, dotv, partials, ctx->xcd_remap, stop, b);
         break;
     case HOP_M:
-        hipLaunchKernelGGL((k_hop_f64<DAG, HOP_M>), dim3(grid), dim3(kBlock), 0, ctx->stream, out,
-                           in, U, g, cb_first, nparity, a, same, dotv, partials, ctx->xcd_remap, stop);
+        hipLaunchKernelGGL((k_hop_f64<DAG, HOP_M>), gd, dim3(kBlock), 0, ctx->stream, out,
+                           in, U, g, cb_first, nparity, a, same, dotv, partials, ctx->xcd_remap, stop, b);
         break;
     default:
-        hipLaunchKernelGGL((k_hop_f64<DAG, HOP_M_DOT>), dim3(grid), dim3(kBlock), 0, ctx->stream,
+        hipLaunchKernelGGL((k_hop_f64<DAG, HOP_M_DOT>), gd, dim3(kBlock), 0, ctx->stream,
                            out, in, U, g, cb_first, nparity, a, same, dotv, partials,
-                           ctx->xcd_remap, stop);
+                           ctx->xcd_remap, stop, b);
         break;
     }
     return wd_check_launch(ctx, "k_hop_f64");
@@ -266,9 +285,24 @@ int wd_launch_hop_f64(wd_ctx *ctx, double2 *out, const double2 *in, const double
                       int cb, int mode, double a, const double2 *same, const double2 *dotv,
                       double2 *partials, const int *stop)
 {
+    const F64Batch one = {0, nullptr, nullptr};
     if (dagger)
-        return launch_mode<true>(ctx, out, in, U, cb, mode, a, same, dotv, partials, stop);
-    return launch_mode<false>(ctx, out, in, U, cb, mode, a, same, dotv, partials, stop);
+        return launch_mode<true>(ctx, out, in, U, cb, mode, a, same, dotv, partials, stop, one, 1);
+    return launch_mode<false>(ctx, out, in, U, cb, mode, a, same, dotv, partials, stop, one, 1);
+}
+
+// nrhs systems at once (blockIdx.y = rhs); b.mask skips systems, b.a_rhs gives
+// per-system diagonal coefficients; partials are [rhs][workgroup].
+int wd_launch_hop_f64_batch(wd_ctx *ctx, double2 *out, const double2 *in, const double2 *U,
+                            int dagger, int cb, int mode, const double2 *same,
+                            const double2 *dotv, double2 *partials, const int *stop,
+                            const F64Batch &b, int nrhs)
+{
+    if (dagger)
+        return launch_mode<true>(ctx, out, in, U, cb, mode, 0.0, same, dotv, partials, stop, b,
+                                 nrhs);
+    return launch_mode<false>(ctx, out, in, U, cb, mode, 0.0, same, dotv, partials, stop, b,
+                              nrhs);
 }
 
 // Number of workgroups a hop launch over `nparity` parities uses (partials count).

@@ -19,7 +19,9 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <map>
 #include <string>
+#include <tuple>
 
 #include "../../include/wilson.h"
 
@@ -47,6 +49,9 @@ struct wd_ctx {
     void *staging;
     size_t staging_bytes;
     int xcd_remap;        // 1: map workgroups so each XCD owns a contiguous site range
+    // autotuned rhs-per-thread of the sloppy operator, keyed by
+    // (nrhs, spinor precision, 100 * gauge precision + recon)
+    std::map<std::tuple<int, int, int>, int> tuned_R;
     std::string last_error;
 };
 
@@ -116,6 +121,26 @@ int wd_launch_hop_f64(wd_ctx *ctx, double2 *out, const double2 *in, const double
                       double2 *partials, const int *stop);
 int wd_launch_zero(wd_ctx *ctx, void *p, size_t bytes);
 int wd_ensure_staging(wd_ctx *ctx, size_t bytes);
+
+// layout / precision conversion (wd_convert.hip)
+void *wd_rhs_ptr(const wd_field *f, int rhs);
+float *wd_rhs_scale(const wd_field *f, int rhs);
+int wd_launch_spinor_from_ref(wd_ctx *ctx, wd_field *f, int rhs, const double2 *src);
+int wd_launch_spinor_to_ref(wd_ctx *ctx, const wd_field *f, int rhs, double2 *dst);
+int wd_launch_gauge_from_ref(wd_ctx *ctx, wd_gauge *U, int mu, const double2 *src);
+int wd_convert_rhs(wd_ctx *ctx, wd_field *dst, int rd, const wd_field *src, int rs);
+
+// batch description for the FP64 kernel: rhs r = blockIdx.y at element offset
+// r * rhs_elems; mask[r] == 0 skips it; a_rhs[r] replaces the diagonal coefficient
+struct F64Batch {
+    size_t rhs_elems;
+    const int *mask;
+    const double *a_rhs;
+};
+int wd_launch_hop_f64_batch(wd_ctx *ctx, double2 *out, const double2 *in, const double2 *U,
+                            int dagger, int cb, int mode, const double2 *same,
+                            const double2 *dotv, double2 *partials, const int *stop,
+                            const F64Batch &b, int nrhs);
 
 // hop modes for the fused FP64 kernel
 enum HopMode {

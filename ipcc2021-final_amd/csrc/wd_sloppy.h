@@ -1,0 +1,31 @@
+// wd_sloppy.h -- arguments and launchers of the FP32-arithmetic multi-rhs
+// operator (wd_hop_sloppy.hip) used by the mixed-precision solvers.
+#pragma once
+
+#include "wd_internal.h"
+
+struct SloppyArgs {
+    char *out;               // field base (rhs 0), spinor format SF
+    float *out_scale;        // F16 scales (rhs 0) or nullptr
+    const char *in;
+    const float *in_scale;
+    const char *same;        // HOP_M / HOP_M_DOT: the diagonal term's input
+    const float *same_scale;
+    const char *dotv;        // HOP_M_DOT: partial of sum conj(dotv) * out
+    const float *dotv_scale;
+    size_t rhs_bytes;        // spinor bytes per rhs (all fields share the format)
+    const char *U;           // gauge field
+    const float *diag;       // [nrhs] a_r = 4 + m_r
+    const int *active;       // [nrhs] 0 = skip this rhs, nullptr = all active
+    double2 *partials;       // [nrhs][nsb] for HOP_M_DOT
+    const int *stop;         // whole launch becomes a no-op when *stop != 0
+    int nrhs;
+    // filled by the launcher
+    int groups, bpp, nsb;
+};
+
+bool wd_sloppy_supported(int spinor_prec, int gauge_prec, int recon);
+int wd_sloppy_groups(int nrhs, int R);
+// cb in {0, 1}: one output parity; cb = -1: both parities.  R in {1, 2, 4}.
+int wd_launch_hop_sloppy(wd_ctx *ctx, SloppyArgs a, int spinor_prec, int gauge_prec, int recon,
+                         int R, int dagger, int cb, int mode);

@@ -40,6 +40,7 @@ EXPORTED = [
     "wd_hop", "wd_diag", "wd_apply_M", "wd_apply_MdagM",
     "wd_norm2", "wd_dot",
     "wd_cg_params_default", "wd_cg", "wd_time_op", "wd_set_option",
+    "wd_mixed_params_default", "wd_cg_mixed", "wd_apply_M_sloppy", "wd_tune_report",
     "wdg_gauge_random_su3", "wdg_gauge_near_identity", "wdg_gauge_unit",
     "wdg_gauge_flip_boundary", "wdg_spinor_gaussian", "wdg_spinor_point", "wdg_philox4x32",
 ]
@@ -72,6 +73,29 @@ class CGReport(ctypes.Structure):
         ("rhs_norm", ctypes.c_double),
         ("seconds", ctypes.c_double),
         ("operator_applies", ctypes.c_long),
+    ]
+
+
+class MixedParams(ctypes.Structure):
+    _fields_ = [
+        ("max_iter", ctypes.c_int),
+        ("tol", ctypes.c_double),
+        ("delta", ctypes.c_double),
+        ("spinor_prec", ctypes.c_int),
+        ("rhs_per_thread", ctypes.c_int),
+        ("check_every", ctypes.c_int),
+        ("use_graph", ctypes.c_int),
+    ]
+
+
+class MixedReport(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_int),
+        ("reliable_updates", ctypes.c_int),
+        ("converged", ctypes.c_int),
+        ("true_rel_rsd", ctypes.c_double),
+        ("rhs_norm", ctypes.c_double),
+        ("rhs_per_thread", ctypes.c_int),
     ]
 
 
@@ -114,6 +138,11 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "wd_dot": ([vp, vp, vp, dp], i),
         "wd_cg_params_default": ([ctypes.POINTER(CGParams)], None),
         "wd_cg": ([vp, vp, vp, vp, d, ctypes.POINTER(CGParams), ctypes.POINTER(CGReport)], i),
+        "wd_mixed_params_default": ([ctypes.POINTER(MixedParams)], None),
+        "wd_cg_mixed": ([vp, vp, vp, vp, vp, dp, ctypes.POINTER(MixedParams),
+                         ctypes.POINTER(MixedReport), ctypes.POINTER(d)], i),
+        "wd_apply_M_sloppy": ([vp, vp, vp, vp, dp, i, i], i),
+        "wd_tune_report": ([vp, vp, vp, ctypes.POINTER(i), dp], i),
         "wd_time_op": ([vp, i, vp, vp, vp, d, i, i, i, ctypes.POINTER(d)], i),
         "wd_set_option": ([vp, i, i], i),
         "wdg_gauge_random_su3": ([ip, ctypes.c_uint64, ctypes.POINTER(dp)], None),
@@ -281,6 +310,43 @@ class Context:
         self.check(self.lib.wd_time_op(self.h, op, out.h, inp.h, U.h, mass, int(dagger), cb,
                                        iters, ctypes.byref(ms)), "wd_time_op")
         return ms.value
+
+    def apply_M_sloppy(self, out, inp, Us, masses, dagger: bool = False, rhs_per_thread: int = 1):
+        m = np.ascontiguousarray(masses, dtype=np.float64)
+        assert m.size == inp.nrhs
+        self.check(self.lib.wd_apply_M_sloppy(self.h, out.h, inp.h, Us.h, _ptr(m), int(dagger),
+                                              rhs_per_thread), "wd_apply_M_sloppy")
+
+    def tune_report(self, f, Us):
+        best = ctypes.c_int()
+        ms = np.zeros(3)
+        self.check(self.lib.wd_tune_report(self.h, f.h, Us.h, ctypes.byref(best), _ptr(ms)),
+                   "wd_tune_report")
+        return best.value, {1: ms[0], 2: ms[1], 4: ms[2]}
+
+    def cg_mixed(self, x, b, U, Us, masses, tol: float = 1e-10, delta: float = 1e-2,
+                 spinor_prec: int = F16, rhs_per_thread: int = 0, max_iter: int = 100000,
+                 check_every: int = 16, use_graph: bool = True):
+        p = MixedParams()
+        self.lib.wd_mixed_params_default(ctypes.byref(p))
+        p.max_iter, p.tol, p.delta = max_iter, tol, delta
+        p.spinor_prec, p.rhs_per_thread = spinor_prec, rhs_per_thread
+        p.check_every, p.use_graph = check_every, int(use_graph)
+        m = np.ascontiguousarray(masses, dtype=np.float64)
+        assert m.size == b.nrhs
+        reps = (MixedReport * b.nrhs)()
+        sec = ctypes.c_double()
+        rc = self.lib.wd_cg_mixed(self.h, x.h, b.h, U.h, Us.h, _ptr(m), ctypes.byref(p), reps,
+                                  ctypes.byref(sec))
+        self.check(rc, "wd_cg_mixed")
+        return {
+            "status": rc,
+            "seconds": sec.value,
+            "systems": [{"iterations": r.iterations, "reliable_updates": r.reliable_updates,
+                         "converged": bool(r.converged), "true_rel_rsd": r.true_rel_rsd,
+                         "rhs_norm": r.rhs_norm, "rhs_per_thread": r.rhs_per_thread}
+                        for r in reps],
+        }
 
     def set_option(self, option: int, value: int):
         self.check(self.lib.wd_set_option(self.h, option, value), "wd_set_option")
