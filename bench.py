@@ -2,27 +2,40 @@
 """Benchmark of the Wilson-Dslash hot path on one MI355X.
 
 A step is one application of the full-lattice Wilson operator M (the
-reference's DslashNew, src/invert.cpp:138-156) to a resident FP64 spinor,
+reference's DslashNew, src/invert.cpp:138-156) to a resident FP64 spinor
 through the fused HIP kernel (both parities, (4+m) diagonal included).
 Workload: 32^3x64, near-identity SU(3) links exp(0.3 i H), Gaussian source,
-kappa 0.135.  Inputs are synthetic (seeded generators) and resident in HBM
-before the timed region.
+kappa 0.135, periodic x/y/z and antiperiodic t.  Inputs are synthetic (seeded
+generators) and resident in HBM before the timed region.
 
-Printed (one JSON line): GFLOP/s of M (1320 hop + 48 diagonal flop per site),
-the roofline of the dominant kernel (algorithmic bytes 8*144 + 3*192 = 1728 B
-per site for M, 8*144 + 2*192 = 1536 B per output site for the one-parity hop)
-timed with HIP events on the library stream, and a CPU baseline: the
-reference's own DslashNew (oracle/_ref, compiled from the reference sources)
-on one host core over a bounded sample, or the oracle restatement when the
-reference build is absent.
+The JSON line carries
+  value          GFLOP/s of M, 1320 (hop) + 48 (diagonal) flop per site
+  roofline       the step's kernel (k_hop_f64, HOP_M): algorithmic bytes
+                 8*144 + 3*192 = 1728 per site / average launch time from HIP
+                 events on the library stream; traffic from the committed
+                 rocprofv3 FETCH_SIZE/WRITE_SIZE summary when present
+  hop_roofline   one-parity FP64 hop (the reference's DslashoffdNew):
+                 1536 B and 1320 flop per output site
+  sloppy_roofline  FP32-arithmetic M on F16 spinors and F32 12-real links:
+                 8*48 + 3*52 = 540 B per site
+  cg             time to a 1e-10 relative residual of M^dag M x = M^dag b:
+                 FP64 CG (the reference's algorithm) vs the mixed solver
+  cpu_baseline   the reference's own DslashNew (compiled from its sources,
+                 oracle/_ref) on one host core over a bounded sample, or the
+                 oracle restatement when that build is absent
+With --workload full it also runs the batched 24^3x48 case (48 point-source
+systems at 4 kappas) and the 48^3x96 case (12 systems, F16 links).
 """
 import argparse
 import ctypes
 import json
 import os
+import re
 import subprocess
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -32,6 +45,8 @@ FLOP_HOP = 1320            # per output site, one hop
 FLOP_M = 1320 + 48         # per site, hop + (4+m) diagonal (fused FMA)
 BYTES_HOP_F64 = 8 * 144 + 2 * 192
 BYTES_M_F64 = 8 * 144 + 3 * 192
+BYTES_M_SLOPPY = 8 * 48 + 3 * 52   # F32 12-real links, F16 spinors + float scale
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_hop_f64_M_32x32x32x64_pmc.txt")
 
 
 def load_pkg():
@@ -39,9 +54,16 @@ def load_pkg():
     return ge.load_package()
 
 
+def log(msg):
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
+def kappa_mass(k):
+    return 1.0 / (2.0 * k) - 4.0
+
+
+# ---- CPU baseline (child process) -------------------------------------------------
 def cpu_baseline_worker(L, kind, seconds):
-    """Runs in a child process: times DslashNew on the host (1 core for the reference)."""
-    import numpy as np
     V = int(np.prod(L))
     rng = np.random.default_rng(20211026)
     src = rng.standard_normal(V * 24)
@@ -49,7 +71,7 @@ def cpu_baseline_worker(L, kind, seconds):
     Up = (ctypes.c_void_p * 4)(*[u.ctypes.data for u in U])
     Lc = (ctypes.c_int * 4)(*L)
     out = np.zeros(V * 24)
-    mass = ctypes.c_double(1.0 / (2 * 0.135) - 4.0)
+    mass = ctypes.c_double(kappa_mass(0.135))
     if kind == "reference":
         lib = ctypes.CDLL(os.path.join(ROOT, "oracle", "_ref", "libref.so"))
         lib.ref_mpi_size()
@@ -71,15 +93,15 @@ def cpu_baseline_worker(L, kind, seconds):
         el = time.perf_counter() - t0
         if el >= seconds:
             break
-    print(json.dumps({"value": FLOP_M * V * n / el / 1e9, "unit": "GFLOP/s", "cores": cores,
-                      "kind": kind, "sample": f"{n} DslashNew applications at "
-                      f"{'x'.join(map(str, L))}, {el:.1f} s"}), flush=True)
+    print(json.dumps({"value": round(FLOP_M * V * n / el / 1e9, 3), "unit": "GFLOP/s",
+                      "cores": cores, "kind": kind,
+                      "sample": f"{n} DslashNew applications (full-lattice M, FP64) at "
+                                f"{'x'.join(map(str, L))} in {el:.1f} s"}), flush=True)
 
 
 def cpu_baseline(L, seconds):
     have_ref = os.path.exists(os.path.join(ROOT, "oracle", "_ref", "libref.so"))
-    kinds = (["reference"] if have_ref else []) + ["port"]
-    for kind in kinds:
+    for kind in (["reference"] if have_ref else []) + ["port"]:
         env = dict(os.environ)
         if kind == "port":
             env["OMP_NUM_THREADS"] = env.get("OMP_NUM_THREADS", "16")
@@ -89,10 +111,140 @@ def cpu_baseline(L, seconds):
                                capture_output=True, text=True, timeout=seconds * 4 + 120, env=env)
             if p.returncode == 0 and p.stdout.strip():
                 return json.loads(p.stdout.strip().splitlines()[-1])
-            print(f"cpu baseline ({kind}) failed: {p.stderr[-500:]}", file=sys.stderr)
+            log(f"cpu baseline ({kind}) failed: {p.stderr[-400:]}")
         except subprocess.TimeoutExpired:
-            print(f"cpu baseline ({kind}) timed out", file=sys.stderr)
+            log(f"cpu baseline ({kind}) timed out")
     return None
+
+
+def pmc_traffic():
+    """HBM bytes per launch of the step's kernel from the committed PMC summary."""
+    if not os.path.exists(PMC_SUMMARY):
+        return None, None
+    text = open(PMC_SUMMARY).read()
+    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", text, re.M)
+    return (float(m.group(1)) if m else None), os.path.relpath(PMC_SUMMARY, ROOT)
+
+
+# ---- GPU workloads -------------------------------------------------------------------
+def cg_compare(w, ctx, U, A, b, mass, max_iter=20000):
+    """FP64 CG (reference algorithm, x0 = 0) vs mixed solver, same system, 1e-10 relative."""
+    fb, x = ctx.field(data=b), ctx.field()
+    fp = ctx.cg(x, fb, U, mass, max_iter=max_iter, tol=1e-10, tol_relative=True, use_x0=False,
+                check_every=16)
+    x64 = x.download()
+    Us = ctx.gauge(A, prec=w.F32, recon=12)
+    xm = ctx.field()
+    mx = ctx.cg_mixed(xm, fb, U, Us, [mass], tol=1e-10, spinor_prec=w.F16, max_iter=max_iter)
+    xmix = xm.download()
+    s = mx["systems"][0]
+    err = float(np.linalg.norm(xmix - x64) / np.linalg.norm(x64))
+    return {
+        "fp64": {"iterations": fp["iterations"], "seconds": round(fp["seconds"], 4),
+                 "solves_per_s": round(1.0 / fp["seconds"], 3),
+                 "true_rel_residual": fp["true_rsd"] / fp["rhs_norm"]},
+        "mixed": {"iterations": s["iterations"], "reliable_updates": s["reliable_updates"],
+                  "seconds": round(mx["seconds"], 4), "solves_per_s": round(1.0 / mx["seconds"], 3),
+                  "true_rel_residual": s["true_rel_rsd"], "rel_error_vs_fp64_solution": err,
+                  "precision": "FP32 arithmetic, F16 spinors, F32 12-real links, FP64 reliable "
+                               "updates"},
+        "speedup_mixed_vs_fp64": round(fp["seconds"] / mx["seconds"], 3),
+    }
+
+
+def batched_case(w):
+    """24^3x48: 12 point sources (spin x colour at a seeded site) x 4 kappas = 48 systems."""
+    L = [24, 24, 24, 48]
+    kappas = [0.120, 0.125, 0.130, 0.135]
+    rng = np.random.RandomState(20211027)
+    site = [int(rng.randint(0, n)) for n in L]
+    A = w.gauge_near_identity(L, 20211025, 0.3)
+    srcs, masses = [], []
+    for k in kappas:
+        for s in range(4):
+            for c in range(3):
+                srcs.append(w.spinor_point(L, site, s, c))
+                masses.append(kappa_mass(k))
+    n = len(srcs)
+    with w.Context(L, bc=[1, 1, 1, -1]) as ctx:
+        U = ctx.gauge(A)
+        Us = ctx.gauge(A, prec=w.F32, recon=12)
+        fb = ctx.field(nrhs=n, data=srcs)
+        x = ctx.field(nrhs=n)
+        log("batched: mixed solve of 48 systems")
+        mx = ctx.cg_mixed(x, fb, U, Us, masses, tol=1e-10, spinor_prec=w.F16)
+        sp = ctx.field(prec=w.F16, nrhs=n, data=srcs)
+        bestR, ms = ctx.tune_report(sp, Us)
+        # FP64 baseline: the reference algorithm, one system after the other
+        log("batched: FP64 CG of the 48 systems one by one")
+        t_fp = 0.0
+        its_fp = []
+        for r in range(n):
+            b1 = ctx.field(data=srcs[r])
+            x1 = ctx.field()
+            res = ctx.cg(x1, b1, U, masses[r], max_iter=20000, tol=1e-10, tol_relative=True,
+                         use_x0=False, check_every=16)
+            t_fp += res["seconds"]
+            its_fp.append(res["iterations"])
+    its = [s["iterations"] for s in mx["systems"]]
+    V = w.volume(L)
+    return {
+        "workload": "24x24x24x48, 12 point sources x kappa {0.120,0.125,0.130,0.135} = 48 systems",
+        "mixed_seconds": round(mx["seconds"], 3), "mixed_solves_per_s": round(n / mx["seconds"], 2),
+        "fp64_sequential_seconds": round(t_fp, 3), "fp64_solves_per_s": round(n / t_fp, 2),
+        "speedup": round(t_fp / mx["seconds"], 3),
+        "all_converged": all(s["converged"] for s in mx["systems"]),
+        "max_true_rel_residual": max(s["true_rel_rsd"] for s in mx["systems"]),
+        "iterations_per_kappa_mixed": [int(np.max(its[12 * k:12 * k + 12])) for k in range(4)],
+        "iterations_per_kappa_fp64": [int(np.max(its_fp[12 * k:12 * k + 12])) for k in range(4)],
+        "rhs_per_thread": mx["systems"][0]["rhs_per_thread"],
+        "sloppy_M_ms_by_rhs_per_thread": {str(k): round(v, 4) for k, v in ms.items()},
+        "sloppy_M_GBps_best": round((8 * 48 / n + 3 * 52) * V * n / (ms[bestR] * 1e-3) / 1e9, 1),
+    }
+
+
+def large_case(w):
+    """48^3x96, 12 Gaussian systems at kappa 0.135, F16 spinors and F16 12-real links."""
+    L = [48, 48, 48, 96]
+    n = 12
+    mass = kappa_mass(0.135)
+    log("large: generating 48^3x96 inputs")
+    A = w.gauge_near_identity(L, 20211025, 0.3)
+    with w.Context(L, bc=[1, 1, 1, -1]) as ctx:
+        U = ctx.gauge(A)
+        Us = ctx.gauge(A, prec=w.F16, recon=12)
+        fb = ctx.field(nrhs=n)
+        for r in range(n):
+            fb.upload(w.spinor_gaussian(L, 20211026, r), r)
+        del A
+        x = ctx.field(nrhs=n)
+        log("large: mixed solve of 12 systems")
+        mx = ctx.cg_mixed(x, fb, U, Us, [mass] * n, tol=1e-10, spinor_prec=w.F16)
+        # FP64 time to solution of one of the systems (all 12 share kappa and source statistics)
+        b1 = ctx.field()
+        b1.upload(w.spinor_gaussian(L, 20211026, 0))
+        x1 = ctx.field()
+        log("large: FP64 CG of one system")
+        fp = ctx.cg(x1, b1, U, mass, max_iter=20000, tol=1e-10, tol_relative=True, use_x0=False,
+                    check_every=16)
+        fin = ctx.field(prec=w.F16, nrhs=n)
+        fin.copy_from(fb)
+        bestR, ms = ctx.tune_report(fin, Us)
+    V = w.volume(L)
+    bytes_site = 8 * 24 / n + 3 * 52   # F16 12-real links shared by n systems, F16 spinors
+    return {
+        "workload": "48x48x48x96, 12 Gaussian systems, kappa 0.135, F16 spinors, F16 12-real links",
+        "mixed_seconds": round(mx["seconds"], 3), "mixed_solves_per_s": round(n / mx["seconds"], 3),
+        "fp64_seconds_per_solve": round(fp["seconds"], 3),
+        "fp64_solves_per_s": round(1.0 / fp["seconds"], 3),
+        "speedup": round(fp["seconds"] * n / mx["seconds"], 3),
+        "all_converged": all(s["converged"] for s in mx["systems"]),
+        "max_true_rel_residual": max(s["true_rel_rsd"] for s in mx["systems"]),
+        "iterations_mixed": max(s["iterations"] for s in mx["systems"]),
+        "iterations_fp64": fp["iterations"],
+        "sloppy_M_ms_by_rhs_per_thread": {str(k): round(v, 3) for k, v in ms.items()},
+        "sloppy_M_GBps_best": round(bytes_site * V * n / (ms[bestR] * 1e-3) / 1e9, 1),
+    }
 
 
 def main():
@@ -102,8 +254,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--lattice", type=int, nargs=4, default=[32, 32, 32, 64])
     ap.add_argument("--kappa", type=float, default=0.135)
+    ap.add_argument("--workload", choices=["default", "full"], default="default")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-cg", action="store_true")
     ap.add_argument("--cpu-worker", default=None)
     args = ap.parse_args()
 
@@ -115,34 +269,51 @@ def main():
               "part of it", file=sys.stderr)
         sys.exit(2)
 
-    pkg = load_pkg()
-    w = pkg.wilson_hip
+    w = load_pkg().wilson_hip
     L = args.lattice
     V = w.volume(L)
-    mass = 1.0 / (2 * args.kappa) - 4.0
-    print(f"bench: generating {'x'.join(map(str, L))} inputs", file=sys.stderr, flush=True)
+    mass = kappa_mass(args.kappa)
+    log(f"generating {'x'.join(map(str, L))} inputs")
     A = w.gauge_near_identity(L, 20211025, 0.3)
     b = w.spinor_gaussian(L, 20211026)
+    extra = {}
     with w.Context(L, bc=[1, 1, 1, -1]) as ctx:
         U = ctx.gauge(A)
-        del A
         fin = ctx.field(data=b)
         fout = ctx.field()
         for _ in range(args.warmup):
             ctx.apply_M(fout, fin, U, mass, False)
         ctx.sync()
+        log(f"timing {args.steps} steps")
         t0 = time.perf_counter()
         for _ in range(args.steps):
             ctx.apply_M(fout, fin, U, mass, False)
         ctx.sync()
         el = time.perf_counter() - t0
         ms_step = el * 1e3 / args.steps
-        # dominant kernel, HIP events on the library stream
-        ms_m = ctx.time_op(w.OP_M, fout, fin, U, mass, False, 0, max(args.steps, 10))
-        ms_hop = ctx.time_op(w.OP_HOP, fout, fin, U, mass, False, 0, max(args.steps, 10))
+        # the step's kernel and the one-parity hop, HIP events on the library stream
+        iters = max(args.steps, 10)
+        ms_m = ctx.time_op(w.OP_M, fout, fin, U, mass, False, 0, iters)
+        ms_hop = ctx.time_op(w.OP_HOP, fout, fin, U, mass, False, 0, iters)
+        # sloppy operator (the mixed solver's inner kernel), one system
+        Us = ctx.gauge(A, prec=w.F32, recon=12)
+        s_in = ctx.field(prec=w.F16, data=b)
+        s_out = ctx.field(prec=w.F16)
+        best_R, ms_s = ctx.tune_report(s_in, Us)
+        if not args.no_cg:
+            log("CG: FP64 vs mixed")
+            extra["cg"] = cg_compare(w, ctx, U, A, b, mass)
+            extra["cg"]["workload"] = (f"{'x'.join(map(str, L))}, kappa {args.kappa}, Gaussian "
+                                       "source, x0 = 0, stop at 1e-10 relative residual")
+        del s_out
+    if args.workload == "full":
+        extra["batched"] = batched_case(w)
+        extra["large"] = large_case(w)
+
     gflops = FLOP_M * V / (ms_step * 1e-3) / 1e9
     gbs_m = BYTES_M_F64 * V / (ms_m * 1e-3) / 1e9
     gbs_hop = BYTES_HOP_F64 * (V // 2) / (ms_hop * 1e-3) / 1e9
+    traffic, traffic_src = pmc_traffic()
     line = {
         "metric": "Dslash GB/s & GFLOP/s vs HBM roofline; CG solves/sec at 1e-10, mixed vs FP64",
         "value": round(gflops, 1),
@@ -155,19 +326,32 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded near-identity SU(3) links, Gaussian spinor)",
+        "data": "synthetic (seeded near-identity SU(3) links exp(0.3iH), Gaussian spinor)",
         "config": {"workload": f"wilson_M_fp64_{'x'.join(map(str, L))}", "lattice": L,
                    "kappa": args.kappa, "bc": "periodic xyz, antiperiodic t",
-                   "step": "one full-lattice M application (fused diag + hop, both parities)"},
+                   "step": "one full-lattice M application (fused diagonal + hop, both parities)"},
         "roofline": {"bound": "hbm", "achieved": round(gbs_m, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(gbs_m / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_hop_f64<HOP_M>", "bytes_per_site": BYTES_M_F64,
+                     "unit": "GB/s", "frac": round(gbs_m / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "kernel": "k_hop_f64<dagger=0, HOP_M>",
+                     "algorithmic_bytes_per_site": BYTES_M_F64, "sites_per_launch": V,
                      "ms_per_launch": round(ms_m, 4)},
-        "hop_roofline": {"achieved": round(gbs_hop, 1), "frac": round(gbs_hop / HBM_PEAK_GBS, 4),
-                         "unit": "GB/s", "bytes_per_site": BYTES_HOP_F64,
-                         "ms_per_launch": round(ms_hop, 4),
+        "hop_roofline": {"kernel": "k_hop_f64<dagger=0, HOP_ONLY>, one parity",
+                         "achieved": round(gbs_hop, 1), "frac": round(gbs_hop / HBM_PEAK_GBS, 4),
+                         "unit": "GB/s", "algorithmic_bytes_per_site": BYTES_HOP_F64,
+                         "sites_per_launch": V // 2, "ms_per_launch": round(ms_hop, 4),
                          "gflops": round(FLOP_HOP * (V // 2) / (ms_hop * 1e-3) / 1e9, 1)},
+        "sloppy_roofline": {"kernel": "k_hop_sloppy<F16 spinors, F32 12-real links, HOP_M>",
+                            "rhs_per_thread": best_R,
+                            "achieved": round(BYTES_M_SLOPPY * V / (ms_s[best_R] * 1e-3) / 1e9, 1),
+                            "frac": round(BYTES_M_SLOPPY * V / (ms_s[best_R] * 1e-3) / 1e9
+                                          / HBM_PEAK_GBS, 4),
+                            "unit": "GB/s", "algorithmic_bytes_per_site": BYTES_M_SLOPPY,
+                            "ms_per_launch": round(ms_s[best_R], 4),
+                            "gflops": round(FLOP_M * V / (ms_s[best_R] * 1e-3) / 1e9, 1)},
     }
+    if traffic_src:
+        line["roofline"]["traffic_source"] = traffic_src
+    line.update(extra)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(L, args.cpu_seconds)
     print(json.dumps(line), flush=True)

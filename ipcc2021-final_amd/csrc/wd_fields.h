@@ -137,8 +137,13 @@ struct Spinor<WD_F16> {
 #pragma unroll
         for (int k = 0; k < 12; ++k)
             m = fmaxf(m, fmaxf(fabsf((float)v[k / 3][k % 3].r), fabsf((float)v[k / 3][k % 3].i)));
-        const float inv = m > 0.f ? 1.0f / m : 0.f;
-        scale[i] = m;
+        // a site whose largest component is below 1e-30 is stored as zero: 1/m
+        // would overflow for denormal m (far from a point source the fields
+        // decay to ~1e-38), turning the zeros into NaN.  NaN inputs still
+        // propagate, so breakdowns stay visible.
+        const bool tiny = m < 1.0e-30f;
+        const float inv = tiny ? 0.f : 1.0f / m;
+        scale[i] = tiny ? 0.f : m;
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
             unsigned w[4];

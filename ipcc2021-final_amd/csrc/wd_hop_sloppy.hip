@@ -102,8 +102,9 @@ __global__ __launch_bounds__(kBlock) void k_hop_sloppy(SloppyArgs a, Geom g, int
     const int lb = logical_block(remap);
     const int grp = lb % a.groups;
     const int sb = lb / a.groups;
-    const int p = (nparity == 2) ? sb / a.bpp : cb_first;
-    const int i = (sb - (nparity == 2 ? p * a.bpp : 0)) * kBlock + threadIdx.x;
+    // both parities alternate over the same site range (shared L2 reuse of in[p])
+    const int p = (nparity == 2) ? (sb & 1) : cb_first;
+    const int i = ((nparity == 2) ? (sb >> 1) : sb) * kBlock + threadIdx.x;
     const int stride = g.stride;
     bool act[R];
     int rhs[R];

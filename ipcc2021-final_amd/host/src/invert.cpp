@@ -27,9 +27,11 @@ int CGinvert(complex<double> *src_p, complex<double> *dest_p, complex<double> *g
 int CGinvert(lattice_fermion &src, lattice_fermion &dest, lattice_gauge &U, const double mass,
              const int max, const double accuracy)
 {
-    wd_ctx *ctx = wdrt::context(src.subgs, src.site_vec);
+    // the start vector comes first: nothing may draw from rand() in between
+    // (creating the GPU context on first use is not guaranteed not to)
     for (int i = 0; i < dest.size; i++)
         dest.A[i] = 1.0 * rand() / RAND_MAX;
+    wd_ctx *ctx = wdrt::context(src.subgs, src.site_vec);
     wdrt::Gauge g(ctx, U);
     wdrt::Field b(ctx, src), x(ctx, dest);
     wd_cg_params p;

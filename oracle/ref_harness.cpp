@@ -16,6 +16,7 @@
 #include <iostream>
 #include <sstream>
 #include <string>
+#include <vector>
 
 #include "check.h"
 #include "dslash.h"
@@ -23,6 +24,7 @@
 #include "invert.h"
 #include "lattice_fermion.h"
 #include "lattice_gauge.h"
+#include "load_gauge.h"
 #include "operator_mpi.h"
 
 typedef std::complex<double> cd;
@@ -226,6 +228,32 @@ void ref_check(const double *src, const double *x, const double *const U[4], con
     }
     _mm_free(s);
     _mm_free(px);
+}
+
+// The steps of the reference's driver (src/main.cpp:77-117) on one rank:
+// srand(seed); source from rand(); load_gauge(gaugefile) ("UNIT", "RANDOM" or a
+// big-endian file); CGinvert(MaxCG = 500, Accuracy = 1e-9); check().  All the
+// text the reference prints is captured into log.
+void ref_main_flow(const int L[4], double mass, const char *gaugefile, unsigned seed, char *log,
+                   long log_len)
+{
+    ensure_mpi();
+    Geometry g(L);
+    captured(log, log_len, [&] {
+        srand(seed);
+        std::vector<double> source(g.V * 24);
+        for (auto &v : source)
+            v = (double)rand() / RAND_MAX;
+        lattice_fermion src(g.subgs, g.site_vec);
+        for (long i = 0; i < g.V * 12; i++)
+            src.A[i] = cd(source[2 * i], source[2 * i + 1]);
+        lattice_gauge U(g.subgs, g.site_vec);
+        std::string fn(gaugefile);
+        load_gauge(U.A, fn, g.subgs, g.site_vec);
+        lattice_fermion dest(g.subgs, g.site_vec);
+        CGinvert(src, dest, U, mass, 500, 1.0e-9);
+        check(src, dest, U, mass);
+    });
 }
 
 } // extern "C"

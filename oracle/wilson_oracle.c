@@ -212,6 +212,17 @@ void or_dot(const double *a, const double *b, long n, double out[2])
     out[1] = im;
 }
 
+/* The reference's CG start vector (src/invert.cpp:42-44): srand(seed), then
+ * x_i = rand()/RAND_MAX + 0i for the n complex entries, from this C library. */
+void or_rand_x0(double *x, long n, unsigned seed)
+{
+    srand(seed);
+    for (long i = 0; i < n; ++i) {
+        x[2 * i] = 1.0 * rand() / RAND_MAX;
+        x[2 * i + 1] = 0.0;
+    }
+}
+
 static cplx cdiv(cplx a, cplx b)
 {
     const double den = b.re * b.re + b.im * b.im;

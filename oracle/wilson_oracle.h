@@ -59,6 +59,10 @@ void or_dot(const double *a, const double *b, long n, double out[2]);
 int or_cg(const double *b, double *x, const double *const U[4], const int L[4], double mass,
           int max, double accuracy, double *rsd_hist);
 
+/* The reference's CG start vector (src/invert.cpp:42-44): srand(seed), then
+ * x_i = rand()/RAND_MAX + 0i, i = 0 .. n-1 (n complex entries). */
+void or_rand_x0(double *x, long n, unsigned seed);
+
 /* Site index helpers (include/operator.h:56-75). */
 long or_site2index(const int x[4], const int L[4]);
 long or_site2eoindex(const int x[4], const int L[4]);

@@ -28,6 +28,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
 import __graft_entry__ as ge  # noqa: E402
+sys.path.insert(0, HERE)
 
 w = ge.load_package().wilson_hip
 REF = ctypes.CDLL(os.path.join(ROOT, "oracle", "_ref", "libref.so"))
@@ -140,6 +141,29 @@ def cg_fixture(name, L, gauge, kappa, seed_g, seed_s, x0_seed, flip_t, max_iter,
             "reference_seconds_1core": round(el, 2)}
 
 
+def main_flow_fixture(name, L, mass, gauge_seed, seed):
+    """The reference's driver steps (src/main.cpp:77-117): rand() source, load_gauge
+    ("UNIT" or a file written from the seeded near-identity generator), CGinvert with
+    MaxCG 500 / Accuracy 1e-9, check().  The printed text is the fixture."""
+    import tempfile
+    from gaugefile import write_gauge_file
+    meta = {"file": name + ".txt", "lattice": L, "mass": mass, "seed": seed}
+    with tempfile.TemporaryDirectory() as td:
+        if gauge_seed is None:
+            gauge = "UNIT"
+        else:
+            A = w.gauge_near_identity(L, gauge_seed, 0.3)
+            gauge = write_gauge_file(os.path.join(td, "gauge.bin"), A, L)
+            meta["gauge_seed"] = gauge_seed
+            meta["gauge_file_sha256"] = hashlib.sha256(open(gauge, "rb").read()).hexdigest()
+        log = ctypes.create_string_buffer(1 << 22)
+        REF.ref_main_flow(_L(L), ctypes.c_double(mass), gauge.encode(), ctypes.c_uint(seed),
+                          log, len(log))
+    open(os.path.join(HERE, name + ".txt"), "w").write(log.value.decode())
+    meta["gauge"] = "UNIT" if gauge_seed is None else "file"
+    return meta
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--cg24", action="store_true", help="also the 24^3x48 CG fixture (slow)")
@@ -161,6 +185,8 @@ def main():
                                        1234, False, 500, 1e-10)
     fx["cg_8x8x8x8_su3_ap"] = cg_fixture("cg_8x8x8x8_su3_ap", [8, 8, 8, 8], "su3", 0.12,
                                          20211025, 20211026, 20211027, True, 1000, 1e-10)
+    fx["main_8x8x8x8_unit"] = main_flow_fixture("main_8x8x8x8_unit", [8, 8, 8, 8], 0.05, None, 0)
+    fx["main_8x8x8x8_file"] = main_flow_fixture("main_8x8x8x8_file", [8, 8, 8, 8], 0.1, 41, 3)
     if args.cg24:
         fx["cg_24x24x24x48_near_ap"] = cg_fixture(
             "cg_24x24x24x48_near_ap", [24, 24, 24, 48], "near", 0.13, 20211025, 20211026,

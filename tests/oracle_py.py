@@ -94,10 +94,9 @@ def kappa_to_mass(kappa):
 def libc_rand_x0(seed, n):
     """The reference's CG start vector: srand(seed); x0_i = rand()/RAND_MAX + 0i
     (src/invert.cpp:42-44), drawn from this process's C library."""
-    libc = ctypes.CDLL(None)
-    libc.srand.argtypes = [ctypes.c_uint]
-    libc.rand.restype = ctypes.c_int
-    libc.srand(seed)
-    RAND_MAX = 2147483647
-    vals = np.array([libc.rand() for _ in range(n)], dtype=np.float64) / RAND_MAX
-    return vals.astype(np.complex128)
+    lib = ctypes.CDLL(ORACLE_LIB)
+    lib.or_rand_x0.argtypes = [ctypes.c_void_p, ctypes.c_long, ctypes.c_uint]
+    lib.or_rand_x0.restype = None
+    x = np.empty(n, dtype=np.complex128)
+    lib.or_rand_x0(_p(x), n, seed)
+    return x

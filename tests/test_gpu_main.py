@@ -1,0 +1,71 @@
+"""End to end through the reference-style C++ API: ipcc2021-final_amd/wilson_main
+(the reference's src/main.cpp flow over libipcc_api.so -> libwilson.so) against
+the text the compiled reference printed for the same arguments
+(tests/golden/main_*.txt): same CG iteration count, the same residual history
+to the 6 digits printed, and the same check() report."""
+import hashlib
+import json
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+GOLD = os.path.join(HERE, "golden")
+MANIFEST = json.load(open(os.path.join(GOLD, "manifest.json")))
+MAINS = [k for k in MANIFEST["fixtures"] if k.startswith("main_")]
+EXE = os.path.join(ROOT, "ipcc2021-final_amd", "wilson_main")
+
+NUM = r"([-+0-9.eEinfa]+)"
+
+
+def parse(text):
+    hist = [float(m.group(1)) for m in re.finditer(r"CG: \d+ iter, rsd \|r\| = " + NUM, text)]
+    conv = re.search(r"CG: (\d+) iterations, convergence residual \|r\| = " + NUM, text)
+    rep = {
+        "b": float(re.search(r"\|b\| = " + NUM, text).group(1)),
+        "x": float(re.search(r"\|x\| = " + NUM, text).group(1)),
+        "MbMb": float(re.search(r"\(Mb, Mb\) = \(" + NUM, text).group(1)),
+        "bMMb": float(re.search(r"\(b, MMb\) = \(" + NUM, text).group(1)),
+        "MxMx": float(re.search(r"\(Mx, Mx\) = \(" + NUM, text).group(1)),
+        "xMMx": float(re.search(r"\(x, MMx\) = \(" + NUM, text).group(1)),
+        "rsd1": float(re.search(r"\|M x -b\| = " + NUM, text).group(1)),
+        "rsd": float(re.search(r"\|M\^dagger M x - Mdb\| = " + NUM, text).group(1)),
+    }
+    return hist, (int(conv.group(1)) if conv else -1), (float(conv.group(2)) if conv else None), rep
+
+
+@pytest.mark.parametrize("name", MAINS)
+def test_wilson_main_matches_reference_driver(name, tmp_path):
+    meta = MANIFEST["fixtures"][name]
+    L = meta["lattice"]
+    gauge = "UNIT"
+    if meta["gauge"] == "file":
+        sys.path.insert(0, GOLD)
+        import __graft_entry__ as ge
+        from gaugefile import write_gauge_file
+        w = ge.load_package().wilson_hip
+        A = w.gauge_near_identity(L, meta["gauge_seed"], 0.3)
+        gauge = write_gauge_file(str(tmp_path / "gauge.bin"), A, L)
+        assert hashlib.sha256(open(gauge, "rb").read()).hexdigest() == meta["gauge_file_sha256"]
+    args = [EXE, repr(meta["mass"]), gauge, *map(str, L), *map(str, L), str(meta["seed"])]
+    print(" ".join(args), flush=True)
+    p = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stderr
+    mine = parse(p.stdout)
+    ref = parse(open(os.path.join(GOLD, meta["file"])).read())
+    print(f"iterations gpu={mine[1]} reference={ref[1]}", flush=True)
+    assert mine[1] == ref[1] and mine[1] > 0
+    assert np.allclose(mine[0], ref[0], rtol=2e-5, atol=0)
+    assert abs(mine[2] - ref[2]) <= 2e-5 * ref[2]
+    for k in ("b", "x", "MbMb", "bMMb", "MxMx", "xMMx"):
+        assert abs(mine[3][k] - ref[3][k]) <= 2e-5 * abs(ref[3][k]), k
+    # the final residuals are at the 1e-10 level and carry rounding noise
+    for k in ("rsd1", "rsd"):
+        assert mine[3][k] < 1e-8 and 0.2 < mine[3][k] / ref[3][k] < 5, k

@@ -142,6 +142,26 @@ def test_mixed_graph_and_direct_launch_agree(w):
         assert out[0][1]["systems"][r]["iterations"] == out[1][1]["systems"][r]["iterations"]
 
 
+def test_f16_storage_of_underflowing_sites(w):
+    """Sites whose components are far below FP32 normal range (as far from a
+    point source) are stored as exact zeros in F16 storage, never as NaN."""
+    L = [4, 4, 4, 4]
+    b = w.spinor_gaussian(L, 3)
+    b[:12] *= 1e-40          # one site entirely denormal in FP32
+    b[12:24] = 0.0           # one exactly zero site
+    A = w.gauge_near_identity(L, 4, 0.3)
+    with w.Context(L) as ctx:
+        f = ctx.field(prec=w.F16, data=b)
+        got = f.download()
+        Us = ctx.gauge(A, prec=w.F32, recon=12)
+        out = ctx.field(prec=w.F16)
+        ctx.apply_M_sloppy(out, f, Us, [0.1], False, 1)
+        Mb = out.download()
+    assert np.all(np.isfinite(got)) and np.all(got[:24] == 0)
+    assert np.all(np.isfinite(Mb))
+    assert rel_l2(got[24:], b[24:]) < 1e-3
+
+
 def test_tune_report(w):
     L = [8, 8, 8, 8]
     _, bs = _setup(w, L, 6)
