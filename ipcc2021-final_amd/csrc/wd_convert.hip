@@ -85,7 +85,7 @@ __global__ __launch_bounds__(kBlock) void k_gauge_from_ref(char *__restrict__ ds
     if constexpr (GF == G_F64_18 || GF == G_F64_12) {
         double2 *b = reinterpret_cast<double2 *>(base);
         for (int k = 0; k < gauge_chunks(GF); ++k)
-            b[(size_t)k * stride + i] = s[k];
+            b[f64_idx(k, (unsigned)i, gauge_chunks(GF))] = s[k];
     } else if constexpr (GF == G_F32_18) {
         float2 *b = reinterpret_cast<float2 *>(base);
         for (int k = 0; k < 9; ++k)

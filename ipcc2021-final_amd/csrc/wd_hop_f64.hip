@@ -14,25 +14,28 @@
 //   HOP_M_DOT HOP_M plus a per-workgroup partial of sum conj(dotv) * out, the
 //             (p, q) product of the CG step fused into q = M^dagger (M p).
 #include "wd_dirac.h"
+#include "wd_fields.h"
 #include "wd_internal.h"
+
+using wdf::f64_idx;
 
 namespace {
 
 typedef double d2v __attribute__((ext_vector_type(2)));
 
-__device__ __forceinline__ cpx<double> ld(const double2 *__restrict__ p, unsigned idx)
+__device__ __forceinline__ cpx<double> ld(const double2 *__restrict__ p, size_t idx)
 {
     const d2v v = *reinterpret_cast<const d2v *>(p + idx);
     return {v.x, v.y};
 }
 
-__device__ __forceinline__ cpx<double> ld_nt(const double2 *__restrict__ p, unsigned idx)
+__device__ __forceinline__ cpx<double> ld_nt(const double2 *__restrict__ p, size_t idx)
 {
     const d2v v = __builtin_nontemporal_load(reinterpret_cast<const d2v *>(p + idx));
     return {v.x, v.y};
 }
 
-__device__ __forceinline__ void st(double2 *__restrict__ p, long idx, cpx<double> z)
+__device__ __forceinline__ void st(double2 *__restrict__ p, size_t idx, cpx<double> z)
 {
     d2v v;
     v.x = z.r;
@@ -49,7 +52,7 @@ __device__ __forceinline__ void load_spinor(const double2 *__restrict__ base, in
     for (int s = 0; s < 4; ++s)
 #pragma unroll
         for (int c = 0; c < 3; ++c)
-            psi[s][c] = ld(base + (size_t)(s * 3 + c) * (unsigned)stride, (unsigned)j);
+            psi[s][c] = ld(base, f64_idx(s * 3 + c, (unsigned)j, 12));
 }
 
 __device__ __forceinline__ void load_link(const double2 *__restrict__ base, int stride, int j,
@@ -59,7 +62,7 @@ __device__ __forceinline__ void load_link(const double2 *__restrict__ base, int 
     for (int a = 0; a < 3; ++a)
 #pragma unroll
         for (int b = 0; b < 3; ++b)
-            U[a][b] = ld_nt(base + (size_t)(a * 3 + b) * (unsigned)stride, (unsigned)j);
+            U[a][b] = ld_nt(base, f64_idx(a * 3 + b, (unsigned)j, 9));
 }
 
 // Forces the accumulator to be materialised here so the compiler cannot
@@ -201,7 +204,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 1))) 
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     res[s][c] = {-0.5 * acc[s][c].r, -0.5 * acc[s][c].i};
-                    st(out_p, (long)(s * 3 + c) * g.stride + i, res[s][c]);
+                    st(out_p, f64_idx(s * 3 + c, (unsigned)i, 12), res[s][c]);
                 }
         } else {
             const double2 *same_p = same + (long)p * 12 * g.stride;
@@ -209,9 +212,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 1))) 
             for (int s = 0; s < 4; ++s)
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    const cpx<double> v = ld(same_p + (size_t)(s * 3 + c) * g.stride, (unsigned)i);
+                    const cpx<double> v = ld(same_p, f64_idx(s * 3 + c, (unsigned)i, 12));
                     res[s][c] = {fma(a, v.r, -0.5 * acc[s][c].r), fma(a, v.i, -0.5 * acc[s][c].i)};
-                    st(out_p, (long)(s * 3 + c) * g.stride + i, res[s][c]);
+                    st(out_p, f64_idx(s * 3 + c, (unsigned)i, 12), res[s][c]);
                 }
         }
     }
@@ -225,7 +228,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 1))) 
             for (int s = 0; s < 4; ++s)
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    const cpx<double> v = ld(dv + (size_t)(s * 3 + c) * g.stride, (unsigned)i);
+                    const cpx<double> v = ld(dv, f64_idx(s * 3 + c, (unsigned)i, 12));
                     pr = fma(v.r, res[s][c].r, pr);
                     pr = fma(v.i, res[s][c].i, pr);
                     pi = fma(v.r, res[s][c].i, pi);

@@ -47,7 +47,14 @@ __device__ __forceinline__ void dir_rhs(cpx<float> (&acc)[R][4][3], const Sloppy
     // forward: U_mu(x) at (parity p, site i); backward: U_mu(x - mu) at (1 - p, j)
     const size_t blk = (size_t)gauge_chunks(GF) * gauge_chunk_bytes(GF) * stride;
     const int lpar = FWD ? p : 1 - p;
-    load_link<GF>(Ubase + (size_t)(MU * 2 + lpar) * blk, stride, (unsigned)(FWD ? i : j), W);
+    // one group: each link is read once, stream it past L2; several groups:
+    // the next groups of this site range re-read it from L2 (uniform branch)
+    if (a.groups > 1)
+        load_link<GF, false>(Ubase + (size_t)(MU * 2 + lpar) * blk, stride, (unsigned)(FWD ? i : j),
+                             W);
+    else
+        load_link<GF, true>(Ubase + (size_t)(MU * 2 + lpar) * blk, stride, (unsigned)(FWD ? i : j),
+                            W);
     using D = Dir<MU, FWD, DAG>;
     const size_t pbytes = (size_t)spinor_chunks(SF) * 16 * stride;
 #pragma unroll

@@ -1,13 +1,15 @@
 // wd_internal.h -- shared definitions of the HIP library behind include/wilson.h.
 //
-// Device storage (all fields even-odd checkerboarded, structure of arrays):
-//   F64 spinor : double2 [rhs][parity][12][stride]        (k = spin*3 + colour)
-//   F32 spinor : float2  [rhs][parity][12][stride]
-//   F16 spinor : half2   [rhs][parity][12][stride] + float scale [rhs][parity][stride]
-//   gauge      : T2      [mu][parity][nrow*3][stride]     (row-major 3x3, nrow = 3 or 2)
-// with stride = Vh rounded up to a multiple of 64, site index i fastest, so a
-// wavefront reading component k of 64 consecutive sites issues one fully
-// coalesced 1 KiB (F64) / 512 B (F32) request.  Padding sites are zero.
+// Device storage (all fields even-odd checkerboarded, site index fastest;
+// wd_fields.h has the exact formats):
+//   F64 spinor : double2, per [rhs][parity] 64-site tiles [stride/64][12][64]
+//   F32 spinor : float4   [rhs][parity][6][stride]  (2 complex per chunk)
+//   F16 spinor : uint4    [rhs][parity][3][stride]  (4 complex as half2) + float scale
+//   gauge      : per [mu][parity]: F64 tiles [stride/64][9 or 6][64], F32 float2[9][stride]
+//                or float4[3][stride], F16 uint2[3][stride]  (row-major 3x3, 12-real = rows 0,1)
+// with stride = Vh rounded up to a multiple of 64, so a wavefront reading one
+// component of 64 consecutive sites issues one coalesced 1 KiB (F64) request.
+// Padding sites are zero.
 //
 // Position i inside a parity block is the reference's eo index within that
 // block: i = xh + Lxh*(y + Ly*(z + Lz*t)), x = 2*xh + ((parity+y+z+t) & 1)

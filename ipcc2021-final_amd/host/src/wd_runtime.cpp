@@ -36,7 +36,16 @@ wd_ctx *context(const int *subgs, const int *site_vec)
         return it->second;
     const char *dev = std::getenv("WD_DEVICE");
     wd_ctx *c = nullptr;
+    // Creating a device context draws from the C library's random() state on
+    // this runtime (observed: a CGinvert whose start vector was drawn after the
+    // first context creation differed from the reference's).  The caller's
+    // rand() sequence is part of the reference's semantics (src/main.cpp:77-85,
+    // src/invert.cpp:42-44), so glibc's state is switched to a scratch array
+    // for the creation and switched back untouched afterwards.
+    static char scratch[256];
+    char *prev = initstate(1u, scratch, sizeof scratch);
     const int rc = wd_create(&c, site_vec, nullptr, dev ? std::atoi(dev) : 0);
+    setstate(prev);
     if (rc != WD_OK)
         throw std::runtime_error(std::string("wd_create: ") + wd_strerror(rc));
     g_ctx[key] = c;
