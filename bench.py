@@ -174,7 +174,7 @@ def batched_case(w):
         log("batched: mixed solve of 48 systems")
         mx = ctx.cg_mixed(x, fb, U, Us, masses, tol=1e-10, spinor_prec=w.F16)
         sp = ctx.field(prec=w.F16, nrhs=n, data=srcs)
-        bestR, ms = ctx.tune_report(sp, Us)
+        bestR, bestC, ms = ctx.tune_report(sp, Us)
         # FP64 baseline: the reference algorithm, one system after the other
         log("batched: FP64 CG of the 48 systems one by one")
         t_fp = 0.0
@@ -198,8 +198,10 @@ def batched_case(w):
         "iterations_per_kappa_mixed": [int(np.max(its[12 * k:12 * k + 12])) for k in range(4)],
         "iterations_per_kappa_fp64": [int(np.max(its_fp[12 * k:12 * k + 12])) for k in range(4)],
         "rhs_per_thread": mx["systems"][0]["rhs_per_thread"],
-        "sloppy_M_ms_by_rhs_per_thread": {str(k): round(v, 4) for k, v in ms.items()},
-        "sloppy_M_GBps_best": round((8 * 48 / n + 3 * 52) * V * n / (ms[bestR] * 1e-3) / 1e9, 1),
+        "site_chunk": mx["systems"][0]["site_chunk"],
+        "sloppy_M_ms_by_R_chunk": {f"{R}/{C}": round(v, 4) for (R, C), v in ms.items()},
+        "sloppy_M_GBps_best": round((8 * 48 / n + 3 * 52) * V * n / (ms[(bestR, bestC)] * 1e-3)
+                                    / 1e9, 1),
     }
 
 
@@ -229,7 +231,7 @@ def large_case(w):
                     check_every=16)
         fin = ctx.field(prec=w.F16, nrhs=n)
         fin.copy_from(fb)
-        bestR, ms = ctx.tune_report(fin, Us)
+        bestR, bestC, ms = ctx.tune_report(fin, Us)
     V = w.volume(L)
     bytes_site = 8 * 24 / n + 3 * 52   # F16 12-real links shared by n systems, F16 spinors
     return {
@@ -242,8 +244,10 @@ def large_case(w):
         "max_true_rel_residual": max(s["true_rel_rsd"] for s in mx["systems"]),
         "iterations_mixed": max(s["iterations"] for s in mx["systems"]),
         "iterations_fp64": fp["iterations"],
-        "sloppy_M_ms_by_rhs_per_thread": {str(k): round(v, 3) for k, v in ms.items()},
-        "sloppy_M_GBps_best": round(bytes_site * V * n / (ms[bestR] * 1e-3) / 1e9, 1),
+        "rhs_per_thread": mx["systems"][0]["rhs_per_thread"],
+        "site_chunk": mx["systems"][0]["site_chunk"],
+        "sloppy_M_ms_by_R_chunk": {f"{R}/{C}": round(v, 3) for (R, C), v in ms.items()},
+        "sloppy_M_GBps_best": round(bytes_site * V * n / (ms[(bestR, bestC)] * 1e-3) / 1e9, 1),
     }
 
 
@@ -299,7 +303,8 @@ def main():
         Us = ctx.gauge(A, prec=w.F32, recon=12)
         s_in = ctx.field(prec=w.F16, data=b)
         s_out = ctx.field(prec=w.F16)
-        best_R, ms_s = ctx.tune_report(s_in, Us)
+        best_R, best_C, ms_t = ctx.tune_report(s_in, Us)
+        ms_s = {R: ms_t[(R, 0)] for R in w.TUNE_R}   # one system: chunk has no effect
         if not args.no_cg:
             log("CG: FP64 vs mixed")
             extra["cg"] = cg_compare(w, ctx, U, A, b, mass)

@@ -158,6 +158,8 @@ typedef struct wd_mixed_params {
     double delta;       /* reliable update when the sloppy |r| < delta x |r| at the last one */
     int spinor_prec;    /* WD_F16 or WD_F32 storage of the inner vectors */
     int rhs_per_thread; /* 1, 2 or 4 systems per thread; 0 = autotune */
+    int site_chunk;     /* site blocks (256 sites) per L2 chunk of the multi-system
+                           operator; 0 = off, -1 = autotune (with rhs_per_thread) */
     int check_every;    /* iterations between host polls of the convergence flags */
     int use_graph;      /* replay each block of check_every iterations as one hipGraph */
 } wd_mixed_params;
@@ -169,10 +171,11 @@ typedef struct wd_mixed_report {
     double true_rel_rsd;  /* |M^dag b - M^dag M x| / |M^dag b| at the last reliable update */
     double rhs_norm;      /* |M^dag b| */
     int rhs_per_thread;   /* the rhs-per-thread the operator ran with */
+    int site_chunk;       /* the L2 chunk the operator ran with */
 } wd_mixed_report;
 
 /* max_iter 100000, tol 1e-10, delta 1e-2, F16 spinors, autotuned rhs per
- * thread, check_every 16, graphs on. */
+ * thread and site chunk, check_every 16, graphs on. */
 void wd_mixed_params_default(wd_mixed_params *p);
 
 /* Solves M_r^dag M_r x_r = M_r^dag b_r for all nrhs systems of b at once
@@ -186,14 +189,23 @@ int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd_gauge *U,
 
 /* out_r = M_r in_r (dagger 0) or M_r^dag in_r with the sloppy operator for all
  * nrhs systems (fields F16 or F32 of the same format, masses[r] host array),
- * rhs_per_thread 1, 2 or 4. */
+ * rhs_per_thread 1, 2 or 4, site_chunk >= 0 (work order only: every chunk
+ * gives bit-identical results). */
 int wd_apply_M_sloppy(wd_ctx *ctx, wd_field *out, const wd_field *in, const wd_gauge *Us,
-                      const double *masses, int dagger, int rhs_per_thread);
+                      const double *masses, int dagger, int rhs_per_thread, int site_chunk);
 
-/* Autotuner: times the sloppy M over f's nrhs systems for 1, 2 and 4 systems
- * per thread (ms[0..2], HIP events on the context stream) and records the
- * fastest (*best_R) for later solves of that shape. */
-int wd_tune_report(wd_ctx *ctx, const wd_field *f, const wd_gauge *Us, int *best_R, double ms[3]);
+/* Candidates of the autotuner: rhs per thread {1, 2, 4} x site chunk
+ * {0, 16, 32, 64}. */
+#define WD_TUNE_NR 3
+#define WD_TUNE_NC 4
+
+/* Autotuner: times the sloppy M over f's nrhs systems for every candidate
+ * (ms[ir * WD_TUNE_NC + ic], HIP events on the context stream; candidates that
+ * cannot differ from chunk 0 because f has one thread group are not timed and
+ * read -1) and records the fastest (*best_R, *best_chunk) for later solves of
+ * that shape. */
+int wd_tune_report(wd_ctx *ctx, const wd_field *f, const wd_gauge *Us, int *best_R,
+                   int *best_chunk, double ms[WD_TUNE_NR * WD_TUNE_NC]);
 
 /* ---- measurement and tuning --------------------------------------------------
  * No counterpart in the reference, which times whole phases with MPI_Wtime
@@ -207,6 +219,14 @@ int wd_time_op(wd_ctx *ctx, int op, wd_field *out, const wd_field *in, const wd_
                double mass, int dagger, int cb, int iters, double *ms_per_launch);
 
 #define WD_OPT_XCD_REMAP 1 /* 1 (default): each XCD processes a contiguous site range */
+/* Work order of the Dslash kernels: 0 = natural site order; W > 0 = z-t tiled
+ * order with columns of W site blocks (256 sites each), so the t neighbours of
+ * a block are read W blocks later instead of one time slice later.  Takes
+ * effect when a time slice of one parity (L0/2 * L1 * L2 sites) is a multiple
+ * of 256 sites and W is below the blocks per slice; else the natural order.
+ * Default: columns of 8 z-planes, W = round(8 * L0/2 * L1 / 256).  Every W
+ * gives bit-identical results. */
+#define WD_OPT_BLOCK_TILE 2
 int wd_set_option(wd_ctx *ctx, int option, int value);
 
 #ifdef __cplusplus

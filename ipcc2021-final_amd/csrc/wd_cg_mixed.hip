@@ -345,51 +345,71 @@ struct FieldBuf {
 } // namespace
 
 // ---- autotuning of the sloppy operator ------------------------------------------
-// Times HOP_M over `nrhs` systems for every rhs-per-thread candidate and keeps
-// the fastest per (nrhs, spinor format, link format).
+// Times HOP_M over `nrhs` systems for every (rhs per thread, site chunk)
+// candidate and keeps the fastest per (nrhs, spinor format, link format).
+// fixed_R > 0 restricts the search to that rhs-per-thread.
+static const int kTuneR[WD_TUNE_NR] = {1, 2, 4};
+static const int kTuneC[WD_TUNE_NC] = {0, 16, 32, 64};
+
 int wd_tune_sloppy(wd_ctx *ctx, const SloppyArgs &a0, int sprec, int gprec, int recon,
-                   int *best_R, double ms_out[3])
+                   int fixed_R, int *best_R, int *best_C, double *ms_out)
 {
     const auto key = std::make_tuple(a0.nrhs, sprec, gprec * 100 + recon);
-    auto it = ctx->tuned_R.find(key);
-    if (it != ctx->tuned_R.end() && !ms_out) {
-        *best_R = it->second;
+    auto it = ctx->tuned.find(key);
+    if (it != ctx->tuned.end() && !ms_out && (fixed_R <= 0 || it->second.first == fixed_R)) {
+        *best_R = it->second.first;
+        *best_C = it->second.second;
         return WD_OK;
     }
-    const int Rs[3] = {1, 2, 4};
+    if (ms_out)
+        for (int k = 0; k < WD_TUNE_NR * WD_TUNE_NC; ++k)
+            ms_out[k] = -1.0;
     double best = 1e30;
-    int bestR = 1;
+    int bestR = fixed_R > 0 ? fixed_R : 1, bestC = 0;
     hipEvent_t e0, e1;
     WD_HIP_CHECK(ctx, hipEventCreate(&e0));
     WD_HIP_CHECK(ctx, hipEventCreate(&e1));
     int rc = WD_OK;
-    for (int k = 0; k < 3 && !rc; ++k) {
-        for (int w = 0; w < 2 && !rc; ++w)
-            rc = wd_launch_hop_sloppy(ctx, a0, sprec, gprec, recon, Rs[k], 0, -1, HOP_M);
-        if (rc)
-            break;
-        (void)hipEventRecord(e0, ctx->stream);
-        const int reps = 3;
-        for (int w = 0; w < reps && !rc; ++w)
-            rc = wd_launch_hop_sloppy(ctx, a0, sprec, gprec, recon, Rs[k], 0, -1, HOP_M);
-        (void)hipEventRecord(e1, ctx->stream);
-        (void)hipEventSynchronize(e1);
-        float ms = 0.f;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        const double t = ms / reps;
-        if (ms_out)
-            ms_out[k] = t;
-        if (t < best) {
-            best = t;
-            bestR = Rs[k];
+    for (int ir = 0; ir < WD_TUNE_NR && !rc; ++ir) {
+        const int R = kTuneR[ir];
+        if (fixed_R > 0 && R != fixed_R)
+            continue;
+        const bool one_group = wd_sloppy_groups(a0.nrhs, R) == 1;
+        for (int ic = 0; ic < WD_TUNE_NC && !rc; ++ic) {
+            if (one_group && ic > 0)
+                break; // the work order cannot change with a single group
+            SloppyArgs a = a0;
+            a.chunk = kTuneC[ic];
+            for (int w = 0; w < 2 && !rc; ++w)
+                rc = wd_launch_hop_sloppy(ctx, a, sprec, gprec, recon, R, 0, -1, HOP_M);
+            if (rc)
+                break;
+            (void)hipEventRecord(e0, ctx->stream);
+            const int reps = 3;
+            for (int w = 0; w < reps && !rc; ++w)
+                rc = wd_launch_hop_sloppy(ctx, a, sprec, gprec, recon, R, 0, -1, HOP_M);
+            (void)hipEventRecord(e1, ctx->stream);
+            (void)hipEventSynchronize(e1);
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            const double t = ms / reps;
+            if (ms_out)
+                ms_out[ir * WD_TUNE_NC + ic] = t;
+            if (t < best) {
+                best = t;
+                bestR = R;
+                bestC = kTuneC[ic];
+            }
         }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc)
         return rc;
-    ctx->tuned_R[key] = bestR;
+    if (fixed_R <= 0)
+        ctx->tuned[key] = std::make_pair(bestR, bestC);
     *best_R = bestR;
+    *best_C = bestC;
     return WD_OK;
 }
 
@@ -402,6 +422,7 @@ extern "C" void wd_mixed_params_default(wd_mixed_params *p)
     p->delta = 1e-2;
     p->spinor_prec = WD_F16;
     p->rhs_per_thread = 0;
+    p->site_chunk = -1;
     p->check_every = 16;
     p->use_graph = 1;
 }
@@ -425,6 +446,7 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
     WD_ARG_CHECK(ctx, prm->rhs_per_thread == 0 || prm->rhs_per_thread == 1 ||
                           prm->rhs_per_thread == 2 || prm->rhs_per_thread == 4,
                  "mixed cg: rhs_per_thread must be 0, 1, 2 or 4");
+    WD_ARG_CHECK(ctx, prm->site_chunk >= -1, "mixed cg: site_chunk must be >= -1");
     WD_ARG_CHECK(ctx, prm->max_iter >= 1 && prm->check_every >= 1 && prm->tol > 0.0 &&
                           prm->delta > 0.0 && prm->delta < 1.0,
                  "mixed cg: bad iteration / tolerance parameters");
@@ -506,15 +528,21 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
     A.stop = stop;
     A.nrhs = n;
     int R = prm->rhs_per_thread;
-    if (R == 0) {
+    int C = prm->site_chunk;
+    if (R == 0 || C < 0) {
         SloppyArgs T = A;
         T.out = (char *)qq.f->data;
         T.out_scale = qq.f->scale;
         T.in = T.same = (const char *)p.f->data;
         T.in_scale = T.same_scale = p.f->scale;
-        if ((rc = wd_tune_sloppy(ctx, T, sp, Us->prec, Us->recon, &R, nullptr)))
+        int tR = 0, tC = 0;
+        if ((rc = wd_tune_sloppy(ctx, T, sp, Us->prec, Us->recon, R, &tR, &tC, nullptr)))
             return rc;
+        R = tR;
+        if (C < 0)
+            C = tC;
     }
+    A.chunk = C;
     const int nsb = hop_nsb;
 
     auto iteration = [&]() -> int {
@@ -659,6 +687,7 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
             reports[r].true_rel_rsd = hs[r].b_norm > 0 ? hs[r].true_rsd / hs[r].b_norm : 0.0;
             reports[r].rhs_norm = hs[r].b_norm;
             reports[r].rhs_per_thread = R;
+            reports[r].site_chunk = C;
         }
         all_conv = all_conv && hs[r].status == 1;
         breakdown = breakdown || hs[r].status == 2;
@@ -669,10 +698,11 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
 }
 
 extern "C" int wd_tune_report(wd_ctx *ctx, const wd_field *f, const wd_gauge *Us, int *best_R,
-                              double ms[3])
+                              int *best_chunk, double ms[WD_TUNE_NR * WD_TUNE_NC])
 {
-    if (!ctx || !f || !Us || !best_R || !ms)
+    if (!ctx)
         return WD_EARG;
+    WD_ARG_CHECK(ctx, f && Us && best_R && best_chunk && ms, "tune: null argument");
     WD_ARG_CHECK(ctx, wd_sloppy_supported(f->prec, Us->prec, Us->recon),
                  "tune: unsupported precision combination");
     wd_field *out = nullptr;
@@ -693,14 +723,14 @@ extern "C" int wd_tune_report(wd_ctx *ctx, const wd_field *f, const wd_gauge *Us
     a.U = (const char *)Us->data;
     a.diag = (const float *)diag.p;
     a.nrhs = f->nrhs;
-    rc = wd_tune_sloppy(ctx, a, f->prec, Us->prec, Us->recon, best_R, ms);
+    rc = wd_tune_sloppy(ctx, a, f->prec, Us->prec, Us->recon, 0, best_R, best_chunk, ms);
     wd_field_destroy(out);
     return rc;
 }
 
 extern "C" int wd_apply_M_sloppy(wd_ctx *ctx, wd_field *out, const wd_field *in,
                                  const wd_gauge *Us, const double *masses, int dagger,
-                                 int rhs_per_thread)
+                                 int rhs_per_thread, int site_chunk)
 {
     if (!ctx)
         return WD_EARG;
@@ -729,6 +759,7 @@ extern "C" int wd_apply_M_sloppy(wd_ctx *ctx, wd_field *out, const wd_field *in,
     a.U = (const char *)Us->data;
     a.diag = (const float *)diag.p;
     a.nrhs = n;
+    a.chunk = site_chunk;
     rc = wd_launch_hop_sloppy(ctx, a, in->prec, Us->prec, Us->recon, rhs_per_thread, dagger, -1,
                               HOP_M);
     if (rc)

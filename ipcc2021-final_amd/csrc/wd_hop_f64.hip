@@ -146,7 +146,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 1))) 
     // range was measured 10 % slower for FP64 on MI355X)
     const int bpp = gridDim.x / nparity;
     const int p = (nparity == 2) ? lb / bpp : cb_first;
-    const int i = (lb - (nparity == 2 ? p * bpp : 0)) * kBlock + threadIdx.x;
+    const int blk = block_order(lb - (nparity == 2 ? p * bpp : 0), g);
+    const int i = blk * kBlock + threadIdx.x;
     const bool active = i < g.Vh;
 
     cpx<double> res[4][3];
@@ -253,7 +254,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 1))) 
                 tr += sr[k];
                 ti += si[k];
             }
-            partials[lb] = make_double2(tr, ti);
+            // natural block order, so the sum does not depend on the work order
+            partials[(nparity == 2 ? p * bpp : 0) + blk] = make_double2(tr, ti);
         }
     }
 }

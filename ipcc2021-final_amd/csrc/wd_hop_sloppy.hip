@@ -13,10 +13,20 @@
 // branch).  Per-rhs diagonal coefficients a_r = 4 + m_r allow several masses
 // in one launch.
 //
+// Instruction budget: with F16 spinors the kernel issues about as many VALU
+// cycles as it moves bytes (measured ~70 % VALU issue on MI355X at one rhs),
+// so the inner loop is written for instruction count: the half spinor is
+// projected straight from the packed halves (one v_fma_mix_f32 per real, no
+// conversions), the site scale, the boundary phase and the -1/2 ride in one
+// factor of the 24 reconstruction FMAs, and loads use a uniform base plus a
+// 32-bit lane offset (no 64-bit address arithmetic).
+//
 // Modes as for the FP64 kernel:
 //   HOP_ONLY  out = -1/2 hop(in)
 //   HOP_M     out = a_r in - 1/2 hop(in)
 //   HOP_M_DOT HOP_M plus per-(rhs, workgroup) FP64 partials of sum conj(dotv) out
+//             (per site in FP32 from the stored F16 values, whose products are
+//             exact in FP32; FP64 across sites)
 #include "wd_fields.h"
 #include "wd_internal.h"
 #include "wd_sloppy.h"
@@ -24,6 +34,133 @@
 using namespace wdf;
 
 namespace {
+
+typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+
+// Real and imaginary part of a complex packed as half2, widened.  Used as
+// operands of fmaf, these become the f16 sources of v_fma_mix_f32 (op_sel
+// picks the half), so no separate conversion instructions are issued.
+__device__ __forceinline__ float re16(unsigned w) { return (float)__builtin_bit_cast(h2v, w).x; }
+__device__ __forceinline__ float im16(unsigned w) { return (float)__builtin_bit_cast(h2v, w).y; }
+
+// Byte offset of 16-byte chunk q of site j in a [chunks][stride] array; the
+// launcher checks that every field fits 32-bit offsets, so loads use a
+// uniform 64-bit base plus a 32-bit lane offset.
+__device__ __forceinline__ unsigned off16(int q, int stride, unsigned j)
+{
+    return ((unsigned)q * (unsigned)stride + j) * 16u;
+}
+
+// The 12 packed complex of an F16 site (component k = 3 * spin + colour).
+__device__ __forceinline__ void load16(const char *base, int stride, unsigned j, unsigned (&w)[12])
+{
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const u4v c = *reinterpret_cast<const u4v *>(base + off16(q, stride, j));
+        w[4 * q] = c.x;
+        w[4 * q + 1] = c.y;
+        w[4 * q + 2] = c.z;
+        w[4 * q + 3] = c.w;
+    }
+}
+
+__device__ __forceinline__ void load32(const char *base, int stride, unsigned j, cpx<float> (&v)[4][3])
+{
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        const f4v c = *reinterpret_cast<const f4v *>(base + off16(q, stride, j));
+        v[(2 * q) / 3][(2 * q) % 3] = {c.x, c.y};
+        v[(2 * q + 1) / 3][(2 * q + 1) % 3] = {c.z, c.w};
+    }
+}
+
+__device__ __forceinline__ float load_scale(const float *base, unsigned j)
+{
+    return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + j * 4u);
+}
+
+// a + mulc<P>(b) from two packed complex, one v_fma_mix_f32 per real.  `one`
+// is 1.0f passed in from the host: with a literal 1 the compiler would fold
+// the fma into widen + add (three instructions instead of one).
+template <int P>
+__device__ __forceinline__ cpx<float> proj16(unsigned wa, unsigned wb, float one)
+{
+    if constexpr (P == 0)
+        return {fmaf(re16(wb), one, re16(wa)), fmaf(im16(wb), one, im16(wa))};
+    else if constexpr (P == 1)
+        return {fmaf(re16(wb), -one, re16(wa)), fmaf(im16(wb), -one, im16(wa))};
+    else if constexpr (P == 2) // +i b = (-b.i, b.r)
+        return {fmaf(im16(wb), -one, re16(wa)), fmaf(re16(wb), one, im16(wa))};
+    else // -i b = (b.i, -b.r)
+        return {fmaf(im16(wb), one, re16(wa)), fmaf(re16(wb), -one, im16(wa))};
+}
+
+// acc += f * mulc<C>(z)
+template <int C>
+__device__ __forceinline__ void acc_mulc(cpx<float> &acc, cpx<float> z, float f)
+{
+    if constexpr (C == 0) {
+        acc.r = fmaf(f, z.r, acc.r);
+        acc.i = fmaf(f, z.i, acc.i);
+    } else if constexpr (C == 1) {
+        acc.r = fmaf(-f, z.r, acc.r);
+        acc.i = fmaf(-f, z.i, acc.i);
+    } else if constexpr (C == 2) {
+        acc.r = fmaf(-f, z.i, acc.r);
+        acc.i = fmaf(f, z.r, acc.i);
+    } else {
+        acc.r = fmaf(f, z.i, acc.r);
+        acc.i = fmaf(-f, z.r, acc.i);
+    }
+}
+
+// acc += f * reconstruct(r) for direction D; f carries -1/2, the boundary
+// phase and the F16 site scale, so the reconstruction is 24 FMAs.
+template <class D>
+__device__ __forceinline__ void reconstruct_f(cpx<float> (&acc)[4][3], const cpx<float> (&r)[2][3],
+                                              float f)
+{
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        acc_mulc<0>(acc[0][c], r[0][c], f);
+        acc_mulc<0>(acc[1][c], r[1][c], f);
+        acc_mulc<D::R0>(acc[D::S0][c], r[0][c], f);
+        acc_mulc<D::R1>(acc[D::S1][c], r[1][c], f);
+    }
+}
+
+// One link with 32-bit lane offsets (row 2 rebuilt for 12-real formats).  NT:
+// non-temporal loads, right when every link is read once per launch.
+template <int GF, bool NT>
+__device__ __forceinline__ void load_link32(const char *base, int stride, unsigned j,
+                                            cpx<float> (&U)[3][3])
+{
+    if constexpr (GF == G_F32_18) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            const f2v c = ldg<NT>(reinterpret_cast<const f2v *>(
+                base + ((unsigned)k * (unsigned)stride + j) * 8u));
+            U[k / 3][k % 3] = {c.x, c.y};
+        }
+    } else if constexpr (GF == G_F32_12) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const f4v c = ldg<NT>(reinterpret_cast<const f4v *>(base + off16(q, stride, j)));
+            U[(2 * q) / 3][(2 * q) % 3] = {c.x, c.y};
+            U[(2 * q + 1) / 3][(2 * q + 1) % 3] = {c.z, c.w};
+        }
+        reconstruct_row2(U);
+    } else {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const u2v c = ldg<NT>(reinterpret_cast<const u2v *>(
+                base + ((unsigned)q * (unsigned)stride + j) * 8u));
+            U[(2 * q) / 3][(2 * q) % 3] = {re16(c.x), im16(c.x)};
+            U[(2 * q + 1) / 3][(2 * q + 1) % 3] = {re16(c.y), im16(c.y)};
+        }
+        reconstruct_row2(U);
+    }
+}
 
 template <int R>
 __device__ __forceinline__ void pin_accs(cpx<float> (&acc)[R][4][3])
@@ -37,39 +174,49 @@ __device__ __forceinline__ void pin_accs(cpx<float> (&acc)[R][4][3])
                 asm volatile("" : "+v"(acc[r][s][c].r), "+v"(acc[r][s][c].i));
 }
 
+// One hop direction for the R right-hand sides of this thread.  in_b / isc_b:
+// per-rhs base of the neighbour parity (spinor data, F16 scales); hs = -1/2
+// times the boundary phase of this hop.
 template <int SF, int GF, int R, int MU, bool FWD, bool DAG>
 __device__ __forceinline__ void dir_rhs(cpx<float> (&acc)[R][4][3], const SloppyArgs &a,
-                                        const bool (&act)[R], const int (&rhs)[R],
-                                        const char *Ubase, int p, int i, int j, float sgn,
-                                        int stride)
+                                        const bool (&act)[R], const char *const (&in_b)[R],
+                                        const float *const (&isc_b)[R], int p, int i, int j,
+                                        float hs, int stride)
 {
     cpx<float> W[3][3];
     // forward: U_mu(x) at (parity p, site i); backward: U_mu(x - mu) at (1 - p, j)
     const size_t blk = (size_t)gauge_chunks(GF) * gauge_chunk_bytes(GF) * stride;
     const int lpar = FWD ? p : 1 - p;
+    const char *Ub = a.U + (size_t)(MU * 2 + lpar) * blk;
     // one group: each link is read once, stream it past L2; several groups:
     // the next groups of this site range re-read it from L2 (uniform branch)
     if (a.groups > 1)
-        load_link<GF, false>(Ubase + (size_t)(MU * 2 + lpar) * blk, stride, (unsigned)(FWD ? i : j),
-                             W);
+        load_link32<GF, false>(Ub, stride, (unsigned)(FWD ? i : j), W);
     else
-        load_link<GF, true>(Ubase + (size_t)(MU * 2 + lpar) * blk, stride, (unsigned)(FWD ? i : j),
-                            W);
+        load_link32<GF, true>(Ub, stride, (unsigned)(FWD ? i : j), W);
     using D = Dir<MU, FWD, DAG>;
-    const size_t pbytes = (size_t)spinor_chunks(SF) * 16 * stride;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         if (!act[r])
             continue;
-        const char *in_r = a.in + (size_t)rhs[r] * a.rhs_bytes + (size_t)(1 - p) * pbytes;
-        const float *sc = a.in_scale ? a.in_scale + (size_t)rhs[r] * 2 * stride +
-                                           (size_t)(1 - p) * stride
-                                     : nullptr;
-        cpx<float> psi[4][3], h[2][3], res[2][3];
-        Spinor<SF>::load(in_r, sc, stride, (unsigned)j, psi);
-        project<D>(psi, h);
+        cpx<float> h[2][3], res[2][3];
+        float f = hs;
+        if constexpr (SF == WD_F16) {
+            unsigned w[12];
+            load16(in_b[r], stride, (unsigned)j, w);
+            f *= load_scale(isc_b[r], (unsigned)j);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                h[0][c] = proj16<D::P0>(w[c], w[3 * D::S0 + c], a.one);
+                h[1][c] = proj16<D::P1>(w[3 + c], w[3 * D::S1 + c], a.one);
+            }
+        } else {
+            cpx<float> psi[4][3];
+            load32(in_b[r], stride, (unsigned)j, psi);
+            project<D>(psi, h);
+        }
         su3_mul_half<!FWD>(W, h, res);
-        reconstruct<D>(acc[r], res, sgn);
+        reconstruct_f<D>(acc[r], res, f);
     }
     pin_accs<R>(acc);
 }
@@ -100,6 +247,49 @@ __device__ __forceinline__ void block_sum_store(double sr, double si, double2 *d
     __syncthreads();
 }
 
+// Stores an F16 site (values / max |component|, scale = that max) and returns
+// the packed words and the scale as stored.
+__device__ __forceinline__ float store16(char *base, float *scale, int stride, unsigned i,
+                                         const cpx<float> (&v)[4][3], unsigned (&w)[12])
+{
+    float m = 0.f;
+#pragma unroll
+    for (int k = 0; k < 12; ++k)
+        m = fmaxf(m, fmaxf(fabsf(v[k / 3][k % 3].r), fabsf(v[k / 3][k % 3].i)));
+    // below 1e-30 the site is stored as zero (1/m would overflow for denormal
+    // m, turning zeros into NaN); NaN inputs still propagate
+    const bool tiny = m < 1.0e-30f;
+    const float inv = tiny ? 0.f : 1.0f / m;
+    const float s = tiny ? 0.f : m;
+    *reinterpret_cast<float *>(reinterpret_cast<char *>(scale) + i * 4u) = s;
+#pragma unroll
+    for (int k = 0; k < 12; ++k)
+        w[k] = f2h(v[k / 3][k % 3].r * inv, v[k / 3][k % 3].i * inv);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        u4v c;
+        c.x = w[4 * q];
+        c.y = w[4 * q + 1];
+        c.z = w[4 * q + 2];
+        c.w = w[4 * q + 3];
+        *reinterpret_cast<u4v *>(base + off16(q, stride, i)) = c;
+    }
+    return s;
+}
+
+__device__ __forceinline__ void store32(char *base, int stride, unsigned i, const cpx<float> (&v)[4][3])
+{
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        f4v c;
+        c.x = v[(2 * q) / 3][(2 * q) % 3].r;
+        c.y = v[(2 * q) / 3][(2 * q) % 3].i;
+        c.z = v[(2 * q + 1) / 3][(2 * q + 1) % 3].r;
+        c.w = v[(2 * q + 1) / 3][(2 * q + 1) % 3].i;
+        *reinterpret_cast<f4v *>(base + off16(q, stride, i)) = c;
+    }
+}
+
 template <int SF, int GF, int R, bool DAG, int MODE>
 __global__ __launch_bounds__(kBlock) void k_hop_sloppy(SloppyArgs a, Geom g, int cb_first,
                                                        int nparity, int remap)
@@ -107,11 +297,28 @@ __global__ __launch_bounds__(kBlock) void k_hop_sloppy(SloppyArgs a, Geom g, int
     if (a.stop && *a.stop)
         return;
     const int lb = logical_block(remap);
-    const int grp = lb % a.groups;
-    const int sb = lb / a.groups;
+    // Work order.  chunk == 0: the groups of one site block are consecutive
+    // (links shared in L2 at once).  chunk == C: C site blocks form a chunk and
+    // the groups take turns over the whole chunk, so between two reads of a
+    // system's t-neighbour only that group's spinors pass through L2 (the
+    // links of the chunk are re-read per group, from L2/MALL).
+    int grp, sb;
+    if (a.chunk > 0 && a.groups > 1) {
+        const int per = a.chunk * a.groups;
+        const int c = lb / per;
+        const int cl = min(a.chunk, a.nsb - c * a.chunk); // last chunk may be short
+        const int rem = lb - c * per;
+        grp = rem / cl;
+        sb = c * a.chunk + (rem - grp * cl);
+    } else {
+        grp = lb % a.groups;
+        sb = lb / a.groups;
+    }
     // both parities alternate over the same site range (shared L2 reuse of in[p])
     const int p = (nparity == 2) ? (sb & 1) : cb_first;
-    const int i = ((nparity == 2) ? (sb >> 1) : sb) * kBlock + threadIdx.x;
+    const int blk = block_order((nparity == 2) ? (sb >> 1) : sb, g);
+    const int nat = (nparity == 2) ? 2 * blk + p : blk; // natural index (partials)
+    const int i = blk * kBlock + threadIdx.x;
     const int stride = g.stride;
     bool act[R];
     int rhs[R];
@@ -125,6 +332,18 @@ __global__ __launch_bounds__(kBlock) void k_hop_sloppy(SloppyArgs a, Geom g, int
     if (!any)
         return; // uniform over the workgroup
     const bool site_ok = i < g.Vh;
+
+    // per-rhs bases (uniform): spinor data and F16 scales of the neighbour parity
+    const size_t pbytes = (size_t)spinor_chunks(SF) * 16 * stride;
+    const char *in_b[R];
+    const float *isc_b[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        in_b[r] = a.in + (size_t)rhs[r] * a.rhs_bytes + (size_t)(1 - p) * pbytes;
+        isc_b[r] = (SF == WD_F16) ? a.in_scale + (size_t)rhs[r] * 2 * stride +
+                                        (size_t)(1 - p) * stride
+                                  : nullptr;
+    }
 
     cpx<float> acc[R][4][3];
 #pragma unroll
@@ -145,46 +364,45 @@ __global__ __launch_bounds__(kBlock) void k_hop_sloppy(SloppyArgs a, Geom g, int
         const int t = rr / g.Lz;
         const int x = 2 * xh + ((p + y + z + t) & 1);
         const int row = i - xh;
-        const float bx = (float)g.bc[0], by = (float)g.bc[1], bz = (float)g.bc[2],
-                    bt = (float)g.bc[3];
-        const char *U = a.U;
+        // -1/2 of the hopping term times the boundary phase of a crossing hop
+        const float hx = -0.5f * (float)g.bc[0], hy = -0.5f * (float)g.bc[1],
+                    hz = -0.5f * (float)g.bc[2], ht = -0.5f * (float)g.bc[3];
         {
             const int xf = (x + 1 == Lx) ? 0 : x + 1, xb = (x == 0) ? Lx - 1 : x - 1;
-            dir_rhs<SF, GF, R, 0, true, DAG>(acc, a, act, rhs, U, p, i, row + (xf >> 1),
-                                             x == Lx - 1 ? bx : 1.f, stride);
-            dir_rhs<SF, GF, R, 0, false, DAG>(acc, a, act, rhs, U, p, i, row + (xb >> 1),
-                                              x == 0 ? bx : 1.f, stride);
+            dir_rhs<SF, GF, R, 0, true, DAG>(acc, a, act, in_b, isc_b, p, i, row + (xf >> 1),
+                                             x == Lx - 1 ? hx : -0.5f, stride);
+            dir_rhs<SF, GF, R, 0, false, DAG>(acc, a, act, in_b, isc_b, p, i, row + (xb >> 1),
+                                              x == 0 ? hx : -0.5f, stride);
         }
         {
             const int s = g.Lxh;
             const int jf = (y + 1 == g.Ly) ? i - (g.Ly - 1) * s : i + s;
             const int jb = (y == 0) ? i + (g.Ly - 1) * s : i - s;
-            dir_rhs<SF, GF, R, 1, true, DAG>(acc, a, act, rhs, U, p, i, jf,
-                                             y == g.Ly - 1 ? by : 1.f, stride);
-            dir_rhs<SF, GF, R, 1, false, DAG>(acc, a, act, rhs, U, p, i, jb, y == 0 ? by : 1.f,
-                                              stride);
+            dir_rhs<SF, GF, R, 1, true, DAG>(acc, a, act, in_b, isc_b, p, i, jf,
+                                             y == g.Ly - 1 ? hy : -0.5f, stride);
+            dir_rhs<SF, GF, R, 1, false, DAG>(acc, a, act, in_b, isc_b, p, i, jb,
+                                              y == 0 ? hy : -0.5f, stride);
         }
         {
             const int s = g.Lxh * g.Ly;
             const int jf = (z + 1 == g.Lz) ? i - (g.Lz - 1) * s : i + s;
             const int jb = (z == 0) ? i + (g.Lz - 1) * s : i - s;
-            dir_rhs<SF, GF, R, 2, true, DAG>(acc, a, act, rhs, U, p, i, jf,
-                                             z == g.Lz - 1 ? bz : 1.f, stride);
-            dir_rhs<SF, GF, R, 2, false, DAG>(acc, a, act, rhs, U, p, i, jb, z == 0 ? bz : 1.f,
-                                              stride);
+            dir_rhs<SF, GF, R, 2, true, DAG>(acc, a, act, in_b, isc_b, p, i, jf,
+                                             z == g.Lz - 1 ? hz : -0.5f, stride);
+            dir_rhs<SF, GF, R, 2, false, DAG>(acc, a, act, in_b, isc_b, p, i, jb,
+                                              z == 0 ? hz : -0.5f, stride);
         }
         {
             const int s = g.Lxh * g.Ly * g.Lz;
             const int jf = (t + 1 == g.Lt) ? i - (g.Lt - 1) * s : i + s;
             const int jb = (t == 0) ? i + (g.Lt - 1) * s : i - s;
-            dir_rhs<SF, GF, R, 3, true, DAG>(acc, a, act, rhs, U, p, i, jf,
-                                             t == g.Lt - 1 ? bt : 1.f, stride);
-            dir_rhs<SF, GF, R, 3, false, DAG>(acc, a, act, rhs, U, p, i, jb, t == 0 ? bt : 1.f,
-                                              stride);
+            dir_rhs<SF, GF, R, 3, true, DAG>(acc, a, act, in_b, isc_b, p, i, jf,
+                                             t == g.Lt - 1 ? ht : -0.5f, stride);
+            dir_rhs<SF, GF, R, 3, false, DAG>(acc, a, act, in_b, isc_b, p, i, jb,
+                                              t == 0 ? ht : -0.5f, stride);
         }
     }
 
-    const size_t pbytes = (size_t)spinor_chunks(SF) * 16 * stride;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         if (!act[r])
@@ -193,50 +411,72 @@ __global__ __launch_bounds__(kBlock) void k_hop_sloppy(SloppyArgs a, Geom g, int
         if (site_ok) {
             const size_t off = (size_t)rhs[r] * a.rhs_bytes + (size_t)p * pbytes;
             const size_t soff = (size_t)rhs[r] * 2 * stride + (size_t)p * stride;
-            cpx<float> res[4][3];
-            if constexpr (MODE == HOP_ONLY) {
-#pragma unroll
-                for (int s = 0; s < 4; ++s)
-#pragma unroll
-                    for (int c = 0; c < 3; ++c)
-                        res[s][c] = {-0.5f * acc[r][s][c].r, -0.5f * acc[r][s][c].i};
-            } else {
-                cpx<float> v[4][3];
-                Spinor<SF>::load(a.same + off, a.same_scale ? a.same_scale + soff : nullptr,
-                                 stride, (unsigned)i, v);
+            const unsigned ui = (unsigned)i;
+            // acc already holds -1/2 hop(in); HOP_M adds a_r * in at the site
+            if constexpr (MODE != HOP_ONLY) {
                 const float ar = a.diag[rhs[r]];
+                if constexpr (SF == WD_F16) {
+                    unsigned v[12];
+                    load16(a.same + off, stride, ui, v);
+                    const float as = ar * load_scale(a.same_scale + soff, ui);
 #pragma unroll
-                for (int s = 0; s < 4; ++s)
-#pragma unroll
-                    for (int c = 0; c < 3; ++c)
-                        res[s][c] = {fmaf(ar, v[s][c].r, -0.5f * acc[r][s][c].r),
-                                     fmaf(ar, v[s][c].i, -0.5f * acc[r][s][c].i)};
-            }
-            Spinor<SF>::store(a.out + off, a.out_scale ? a.out_scale + soff : nullptr, stride,
-                              (unsigned)i, res);
-            if constexpr (MODE == HOP_M_DOT) {
-                // the partial uses the stored (rounded) values, as a later
-                // reduction over the field would
-                Spinor<SF>::load(a.out + off, a.out_scale ? a.out_scale + soff : nullptr,
-                                 stride, (unsigned)i, res);
-                cpx<float> d[4][3];
-                Spinor<SF>::load(a.dotv + off, a.dotv_scale ? a.dotv_scale + soff : nullptr,
-                                 stride, (unsigned)i, d);
-#pragma unroll
-                for (int s = 0; s < 4; ++s)
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        const double dr = d[s][c].r, di = d[s][c].i;
-                        const double qr = res[s][c].r, qi = res[s][c].i;
-                        pr = fma(dr, qr, pr);
-                        pr = fma(di, qi, pr);
-                        pi = fma(dr, qi, pi);
-                        pi = fma(-di, qr, pi);
+                    for (int k = 0; k < 12; ++k) {
+                        acc[r][k / 3][k % 3].r = fmaf(re16(v[k]), as, acc[r][k / 3][k % 3].r);
+                        acc[r][k / 3][k % 3].i = fmaf(im16(v[k]), as, acc[r][k / 3][k % 3].i);
                     }
+                } else {
+                    cpx<float> v[4][3];
+                    load32(a.same + off, stride, ui, v);
+#pragma unroll
+                    for (int s = 0; s < 4; ++s)
+#pragma unroll
+                        for (int c = 0; c < 3; ++c)
+                            acc[r][s][c] = {fmaf(ar, v[s][c].r, acc[r][s][c].r),
+                                            fmaf(ar, v[s][c].i, acc[r][s][c].i)};
+                }
+            }
+            if constexpr (SF == WD_F16) {
+                unsigned q[12];
+                const float sq = store16(a.out + off, a.out_scale + soff, stride, ui, acc[r], q);
+                if constexpr (MODE == HOP_M_DOT) {
+                    // sum conj(d) q over the stored (rounded) values: f16 x f16
+                    // products are exact in FP32, scales applied once
+                    unsigned d[12];
+                    load16(a.dotv + off, stride, ui, d);
+                    const float sd = load_scale(a.dotv_scale + soff, ui);
+                    float tr = 0.f, ti = 0.f;
+#pragma unroll
+                    for (int k = 0; k < 12; ++k) {
+                        tr = fmaf(re16(d[k]), re16(q[k]), tr);
+                        tr = fmaf(im16(d[k]), im16(q[k]), tr);
+                        ti = fmaf(re16(d[k]), im16(q[k]), ti);
+                        ti = fmaf(-im16(d[k]), re16(q[k]), ti);
+                    }
+                    const double ss = (double)sd * (double)sq;
+                    pr = (double)tr * ss;
+                    pi = (double)ti * ss;
+                }
+            } else {
+                store32(a.out + off, stride, ui, acc[r]);
+                if constexpr (MODE == HOP_M_DOT) {
+                    cpx<float> d[4][3];
+                    load32(a.dotv + off, stride, ui, d);
+#pragma unroll
+                    for (int s = 0; s < 4; ++s)
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) {
+                            const double dr = d[s][c].r, di = d[s][c].i;
+                            const double qr = acc[r][s][c].r, qi = acc[r][s][c].i;
+                            pr = fma(dr, qr, pr);
+                            pr = fma(di, qi, pr);
+                            pi = fma(dr, qi, pi);
+                            pi = fma(-di, qr, pi);
+                        }
+                }
             }
         }
         if constexpr (MODE == HOP_M_DOT)
-            block_sum_store(pr, pi, a.partials + (size_t)rhs[r] * a.nsb + sb);
+            block_sum_store(pr, pi, a.partials + (size_t)rhs[r] * a.nsb + nat);
     }
 }
 
@@ -310,6 +550,10 @@ int wd_launch_hop_sloppy(wd_ctx *ctx, SloppyArgs a, int spinor_prec, int gauge_p
 {
     const Geom &g = ctx->geom;
     WD_ARG_CHECK(ctx, R == 1 || R == 2 || R == 4, "sloppy hop: rhs per thread must be 1, 2 or 4");
+    WD_ARG_CHECK(ctx, a.chunk >= 0, "sloppy hop: site chunk must be >= 0");
+    // every per-parity array (at most 6 x 16 bytes per site) must fit 32-bit offsets
+    WD_ARG_CHECK(ctx, 96.0 * g.stride < 4294967296.0, "sloppy hop: lattice too large");
+    a.one = 1.0f;
     WD_ARG_CHECK(ctx, wd_sloppy_supported(spinor_prec, gauge_prec, recon),
                  "sloppy hop: unsupported precision combination");
     const int nparity = (cb < 0) ? 2 : 1;

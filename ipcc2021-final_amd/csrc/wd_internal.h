@@ -32,6 +32,8 @@ struct Geom {
     int Vh;              // sites per parity
     int stride;          // component stride (>= Vh, multiple of 64)
     double bc[4];        // boundary phase applied to hops that cross the boundary in mu
+    int tD;              // site blocks per time slice of one parity (0: slices not block aligned)
+    int tW;              // block column width of the z-t tiled order (0: natural order)
 };
 
 struct wd_ctx {
@@ -51,9 +53,9 @@ struct wd_ctx {
     void *staging;
     size_t staging_bytes;
     int xcd_remap;        // 1: map workgroups so each XCD owns a contiguous site range
-    // autotuned rhs-per-thread of the sloppy operator, keyed by
+    // autotuned (rhs per thread, chunk) of the sloppy operator, keyed by
     // (nrhs, spinor precision, 100 * gauge precision + recon)
-    std::map<std::tuple<int, int, int>, int> tuned_R;
+    std::map<std::tuple<int, int, int>, std::pair<int, int>> tuned;
     std::string last_error;
 };
 
@@ -109,6 +111,27 @@ __device__ __forceinline__ int logical_block(int remap)
     if (remap && (n % kNumXCD) == 0)
         return (b % kNumXCD) * (n / kNumXCD) + b / kNumXCD;
     return b;
+}
+
+// Order of the site blocks of one parity.  Blocks of kBlock consecutive sites
+// advance in x, then y, z, t; with D blocks per time slice the t neighbours of
+// block b sit in blocks b -+ D, which the natural order reaches D blocks later,
+// after the L2 has been refilled with links when D is large (64 blocks at
+// 32^3 x 64).  In the tiled order the blocks run in columns of W: the first W
+// blocks of slice 0, the same W blocks of slice 1, ..., of slice Lt-1, then
+// the next W blocks of every slice.  A t neighbour is then W blocks away and
+// only the z neighbours across a column edge leave the working set.  Returns
+// the natural index of logical block lb; a bijection on [0, D * Lt).
+__device__ __forceinline__ int block_order(int lb, const Geom &g)
+{
+    if (g.tW <= 0)
+        return lb;
+    const int per = g.tW * g.Lt;
+    const int col = lb / per;
+    const int w = min(g.tW, g.tD - col * g.tW); // the last column may be narrower
+    const int rem = lb - col * per;
+    const int t = rem / w;
+    return col * g.tW + (rem - t * w) + g.tD * t;
 }
 
 // ---- precision helpers ---------------------------------------------------------

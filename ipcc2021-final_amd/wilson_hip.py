@@ -29,7 +29,12 @@ WD_ENOTCONV = 1
 F64, F32, F16 = 0, 1, 2
 OP_HOP, OP_M = 0, 1
 OPT_XCD_REMAP = 1
+OPT_BLOCK_TILE = 2
 RECON_18, RECON_12 = 18, 12
+# autotuner candidates (include/wilson.h WD_TUNE_NR x WD_TUNE_NC)
+TUNE_R = (1, 2, 4)
+TUNE_CHUNK = (0, 16, 32, 64)
+TUNE_NR, TUNE_NC = len(TUNE_R), len(TUNE_CHUNK)
 
 # Every symbol include/wilson.h and include/wilson_gen.h declare.
 EXPORTED = [
@@ -83,6 +88,7 @@ class MixedParams(ctypes.Structure):
         ("delta", ctypes.c_double),
         ("spinor_prec", ctypes.c_int),
         ("rhs_per_thread", ctypes.c_int),
+        ("site_chunk", ctypes.c_int),
         ("check_every", ctypes.c_int),
         ("use_graph", ctypes.c_int),
     ]
@@ -96,6 +102,7 @@ class MixedReport(ctypes.Structure):
         ("true_rel_rsd", ctypes.c_double),
         ("rhs_norm", ctypes.c_double),
         ("rhs_per_thread", ctypes.c_int),
+        ("site_chunk", ctypes.c_int),
     ]
 
 
@@ -141,8 +148,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "wd_mixed_params_default": ([ctypes.POINTER(MixedParams)], None),
         "wd_cg_mixed": ([vp, vp, vp, vp, vp, dp, ctypes.POINTER(MixedParams),
                          ctypes.POINTER(MixedReport), ctypes.POINTER(d)], i),
-        "wd_apply_M_sloppy": ([vp, vp, vp, vp, dp, i, i], i),
-        "wd_tune_report": ([vp, vp, vp, ctypes.POINTER(i), dp], i),
+        "wd_apply_M_sloppy": ([vp, vp, vp, vp, dp, i, i, i], i),
+        "wd_tune_report": ([vp, vp, vp, ctypes.POINTER(i), ctypes.POINTER(i), dp], i),
         "wd_time_op": ([vp, i, vp, vp, vp, d, i, i, i, ctypes.POINTER(d)], i),
         "wd_set_option": ([vp, i, i], i),
         "wdg_gauge_random_su3": ([ip, ctypes.c_uint64, ctypes.POINTER(dp)], None),
@@ -311,26 +318,33 @@ class Context:
                                        iters, ctypes.byref(ms)), "wd_time_op")
         return ms.value
 
-    def apply_M_sloppy(self, out, inp, Us, masses, dagger: bool = False, rhs_per_thread: int = 1):
+    def apply_M_sloppy(self, out, inp, Us, masses, dagger: bool = False, rhs_per_thread: int = 1,
+                       site_chunk: int = 0):
         m = np.ascontiguousarray(masses, dtype=np.float64)
         assert m.size == inp.nrhs
         self.check(self.lib.wd_apply_M_sloppy(self.h, out.h, inp.h, Us.h, _ptr(m), int(dagger),
-                                              rhs_per_thread), "wd_apply_M_sloppy")
+                                              rhs_per_thread, site_chunk), "wd_apply_M_sloppy")
 
     def tune_report(self, f, Us):
-        best = ctypes.c_int()
-        ms = np.zeros(3)
-        self.check(self.lib.wd_tune_report(self.h, f.h, Us.h, ctypes.byref(best), _ptr(ms)),
-                   "wd_tune_report")
-        return best.value, {1: ms[0], 2: ms[1], 4: ms[2]}
+        """Times every (rhs per thread, site chunk) candidate of the sloppy M over
+        f's systems; returns (best_R, best_chunk, {(R, chunk): ms}) without the
+        candidates that were not timed."""
+        best, chunk = ctypes.c_int(), ctypes.c_int()
+        ms = np.zeros(TUNE_NR * TUNE_NC)
+        self.check(self.lib.wd_tune_report(self.h, f.h, Us.h, ctypes.byref(best),
+                                           ctypes.byref(chunk), _ptr(ms)), "wd_tune_report")
+        table = {(R, C): ms[ir * TUNE_NC + ic]
+                 for ir, R in enumerate(TUNE_R) for ic, C in enumerate(TUNE_CHUNK)
+                 if ms[ir * TUNE_NC + ic] >= 0}
+        return best.value, chunk.value, table
 
     def cg_mixed(self, x, b, U, Us, masses, tol: float = 1e-10, delta: float = 1e-2,
-                 spinor_prec: int = F16, rhs_per_thread: int = 0, max_iter: int = 100000,
-                 check_every: int = 16, use_graph: bool = True):
+                 spinor_prec: int = F16, rhs_per_thread: int = 0, site_chunk: int = -1,
+                 max_iter: int = 100000, check_every: int = 16, use_graph: bool = True):
         p = MixedParams()
         self.lib.wd_mixed_params_default(ctypes.byref(p))
         p.max_iter, p.tol, p.delta = max_iter, tol, delta
-        p.spinor_prec, p.rhs_per_thread = spinor_prec, rhs_per_thread
+        p.spinor_prec, p.rhs_per_thread, p.site_chunk = spinor_prec, rhs_per_thread, site_chunk
         p.check_every, p.use_graph = check_every, int(use_graph)
         m = np.ascontiguousarray(masses, dtype=np.float64)
         assert m.size == b.nrhs
@@ -344,7 +358,8 @@ class Context:
             "seconds": sec.value,
             "systems": [{"iterations": r.iterations, "reliable_updates": r.reliable_updates,
                          "converged": bool(r.converged), "true_rel_rsd": r.true_rel_rsd,
-                         "rhs_norm": r.rhs_norm, "rhs_per_thread": r.rhs_per_thread}
+                         "rhs_norm": r.rhs_norm, "rhs_per_thread": r.rhs_per_thread,
+                         "site_chunk": r.site_chunk}
                         for r in reps],
         }
 

@@ -60,6 +60,30 @@ def test_rhs_per_thread_variants_agree_bitwise(w):
             assert np.array_equal(outs[R][r], outs[1][r])
 
 
+def test_site_chunk_orders_agree_bitwise(w):
+    # 8^4: 2048 sites per parity = 8 site blocks, 16 over both parities; chunks
+    # of 3 blocks leave a short last chunk, 1 is the finest order, 64 > 16 a
+    # single chunk.  The work order must not change a single bit.
+    L = [8, 8, 8, 8]
+    A, bs = _setup(w, L, 7)
+    masses = [0.1, -0.1, 0.2, 0.0, 0.05, 0.3, -0.2]
+    with w.Context(L) as ctx:
+        Us = ctx.gauge(A, prec=w.F32, recon=12)
+        fin = ctx.field(prec=w.F16, nrhs=7, data=bs)
+        for dag in (False, True):
+            ref = None
+            for R in (1, 2, 4):
+                for C in (0, 1, 3, 64):
+                    fout = ctx.field(prec=w.F16, nrhs=7)
+                    ctx.apply_M_sloppy(fout, fin, Us, masses, dag, R, C)
+                    got = [fout.download(r) for r in range(7)]
+                    if ref is None:
+                        ref = got
+                        continue
+                    for r in range(7):
+                        assert np.array_equal(got[r], ref[r]), (dag, R, C, r)
+
+
 def _true_rel_residual(oracle, b, x, A, L, mass):
     Mdb = oracle.dslash(b, A, L, mass, 1)
     r = Mdb - oracle.dslash(oracle.dslash(x, A, L, mass, 0), A, L, mass, 1)
@@ -169,9 +193,17 @@ def test_tune_report(w):
     with w.Context(L) as ctx:
         Us = ctx.gauge(A, prec=w.F32, recon=12)
         f = ctx.field(prec=w.F16, nrhs=6, data=bs)
-        best, ms = ctx.tune_report(f, Us)
-    assert best in (1, 2, 4) and all(v > 0 for v in ms.values())
-    assert ms[best] == min(ms.values())
+        best, chunk, ms = ctx.tune_report(f, Us)
+        # 6 systems: R = 1, 2, 4 all leave more than one group, so every
+        # candidate is timed
+        assert set(ms) == {(R, C) for R in w.TUNE_R for C in w.TUNE_CHUNK}
+        assert best in w.TUNE_R and chunk in w.TUNE_CHUNK
+        assert all(v > 0 for v in ms.values())
+        assert ms[(best, chunk)] == min(ms.values())
+        f1 = ctx.field(prec=w.F16, nrhs=1, data=bs[:1])
+        _, chunk1, ms1 = ctx.tune_report(f1, Us)
+        # one system: a single group for every R, chunk candidates not timed
+        assert set(ms1) == {(R, 0) for R in w.TUNE_R} and chunk1 == 0
 
 
 def test_sloppy_argument_errors(w):
@@ -188,3 +220,5 @@ def test_sloppy_argument_errors(w):
         o16 = ctx.field(prec=w.F16, nrhs=2)
         with pytest.raises(w.WilsonError):
             ctx.apply_M_sloppy(o16, f16, Us, [0.1, 0.1], rhs_per_thread=3)
+        with pytest.raises(w.WilsonError):
+            ctx.apply_M_sloppy(o16, f16, Us, [0.1, 0.1], site_chunk=-1)
