@@ -146,6 +146,23 @@ void wd_cg_params_default(wd_cg_params *p);
 int wd_cg(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd_gauge *U, double mass,
           const wd_cg_params *params, wd_cg_report *report);
 
+/* Schur even-odd preconditioned CGNR, FP64.  No counterpart in the reference's
+ * code: the problem statement (doc/IPCC_2021.pdf p.11-12) describes the Schur
+ * complement, the reference solves the full system only.  The even half solves
+ * A x_e = a b_e + D_eo b_o / 2 with A = a^2 - D_eo D_oe / 4 (a = 4 + m) by CGNR
+ * on A^dagger A; then x_o = (b_o + D_oe x_e / 2) / a.  Same parameters and
+ * report as wd_cg; the stop test is the reference's, on the FP64 residual
+ * |M^dagger b - M^dagger M x| of the full system (params->tol, relative or
+ * absolute), checked after the reconstruction (CG resumes from x_e with a
+ * tighter Schur threshold if it was not met, up to 3 times).  report->rsd is
+ * the last Schur residual, iterations count Schur iterations, operator_applies
+ * counts applications of A, A^dagger, M or M^dagger (each costs one
+ * full-lattice hop).  rsd_history receives the Schur residual per iteration.
+ * The solution equals wd_cg's to the tolerance, not bitwise; iteration counts
+ * are lower. */
+int wd_cg_eo(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd_gauge *U, double mass,
+             const wd_cg_params *params, wd_cg_report *report);
+
 /* ---- mixed precision and many right-hand sides ----------------------------------
  * No counterpart in the reference (its only solver is the FP64 CGinvert,
  * src/invert.cpp:25-121).  The inner iterations use FP32 arithmetic on

@@ -80,14 +80,16 @@ __global__ __launch_bounds__(kBlock) void k_mx_xpay(char *__restrict__ p, float 
     const size_t so = ((size_t)rhs * 2 + par) * stride;
     const float beta = (float)st[rhs].beta;
     cpx<float> vp[4][3], vr[4][3];
-    Spinor<SF>::load(p + off, ps ? ps + so : nullptr, stride, (unsigned)i, vp);
-    Spinor<SF>::load(r + off, rs ? rs + so : nullptr, stride, (unsigned)i, vr);
+    // streamed once per iteration: non-temporal
+    Spinor<SF>::template load<float, true>(p + off, ps ? ps + so : nullptr, stride, (unsigned)i, vp);
+    Spinor<SF>::template load<float, true>(r + off, rs ? rs + so : nullptr, stride, (unsigned)i, vr);
 #pragma unroll
     for (int s = 0; s < 4; ++s)
 #pragma unroll
         for (int c = 0; c < 3; ++c)
             vp[s][c] = {fmaf(beta, vp[s][c].r, vr[s][c].r), fmaf(beta, vp[s][c].i, vr[s][c].i)};
-    Spinor<SF>::store(p + off, ps ? ps + so : nullptr, stride, (unsigned)i, vp);
+    Spinor<SF>::template store<float, true>(p + off, ps ? ps + so : nullptr, stride, (unsigned)i,
+                                            vp);
 }
 
 template <int SF>
@@ -109,10 +111,14 @@ __global__ __launch_bounds__(kBlock) void k_mx_update(
         const size_t so = ((size_t)rhs * 2 + par) * stride;
         const float alpha = (float)st[rhs].alpha;
         cpx<float> vx[4][3], vr[4][3], vp[4][3], vq[4][3];
-        Spinor<WD_F32>::load(xs + xoff, nullptr, stride, (unsigned)i, vx);
-        Spinor<SF>::load(r + off, rs ? rs + so : nullptr, stride, (unsigned)i, vr);
-        Spinor<SF>::load(p + off, ps ? ps + so : nullptr, stride, (unsigned)i, vp);
-        Spinor<SF>::load(q + off, qs ? qs + so : nullptr, stride, (unsigned)i, vq);
+        // streamed once per iteration: non-temporal
+        Spinor<WD_F32>::template load<float, true>(xs + xoff, nullptr, stride, (unsigned)i, vx);
+        Spinor<SF>::template load<float, true>(r + off, rs ? rs + so : nullptr, stride,
+                                               (unsigned)i, vr);
+        Spinor<SF>::template load<float, true>(p + off, ps ? ps + so : nullptr, stride,
+                                               (unsigned)i, vp);
+        Spinor<SF>::template load<float, true>(q + off, qs ? qs + so : nullptr, stride,
+                                               (unsigned)i, vq);
 #pragma unroll
         for (int s = 0; s < 4; ++s)
 #pragma unroll
@@ -122,17 +128,23 @@ __global__ __launch_bounds__(kBlock) void k_mx_update(
                 vr[s][c] = {fmaf(-alpha, vq[s][c].r, vr[s][c].r),
                             fmaf(-alpha, vq[s][c].i, vr[s][c].i)};
             }
-        Spinor<WD_F32>::store(xs + xoff, nullptr, stride, (unsigned)i, vx);
-        Spinor<SF>::store(r + off, rs ? rs + so : nullptr, stride, (unsigned)i, vr);
+        Spinor<WD_F32>::template store<float, true>(xs + xoff, nullptr, stride, (unsigned)i, vx);
         // |r_s|^2 of the stored (rounded) residual
-        Spinor<SF>::load(r + off, rs ? rs + so : nullptr, stride, (unsigned)i, vr);
+        if constexpr (SF == WD_F16) {
+            unsigned w[12];
+            const float sc = Spinor<WD_F16>::template store_packed<float, true>(
+                r + off, rs + so, stride, (unsigned)i, vr, w);
+            acc = norm2_packed(w, sc);
+        } else {
+            Spinor<SF>::template store<float, true>(r + off, nullptr, stride, (unsigned)i, vr);
 #pragma unroll
-        for (int s = 0; s < 4; ++s)
+            for (int s = 0; s < 4; ++s)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                acc = fma((double)vr[s][c].r, (double)vr[s][c].r, acc);
-                acc = fma((double)vr[s][c].i, (double)vr[s][c].i, acc);
-            }
+                for (int c = 0; c < 3; ++c) {
+                    acc = fma((double)vr[s][c].r, (double)vr[s][c].r, acc);
+                    acc = fma((double)vr[s][c].i, (double)vr[s][c].i, acc);
+                }
+        }
     }
     block_reduce_store(acc, 0.0, partials + (size_t)rhs * gridDim.x + blockIdx.x);
 }
@@ -492,8 +504,8 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
     const size_t xe = x->rhs_elems;
     const size_t srb = rs.f->rhs_elems * prec_bytes_complex(sp);
     const size_t xsb = xs.f->rhs_elems * prec_bytes_complex(WD_F32);
-    F64Batch all = {xe, nullptr, (const double *)diag_d.p};
-    F64Batch rel = {xe, reliable, (const double *)diag_d.p};
+    F64Batch all = {xe, nullptr, (const double *)diag_d.p, nullptr};
+    F64Batch rel = {xe, reliable, (const double *)diag_d.p, nullptr};
 
     // Mdb = M^dag b for every system; x = 0; r_s = Mdb; rho = |Mdb|^2
     if ((rc = wd_launch_hop_f64_batch(ctx, (double2 *)Mdb.f->data, (const double2 *)b->data, Ud, 1,
@@ -527,6 +539,7 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
     A.partials = part;
     A.stop = stop;
     A.nrhs = n;
+    A.hc = -0.5f;
     int R = prm->rhs_per_thread;
     int C = prm->site_chunk;
     if (R == 0 || C < 0) {
@@ -723,6 +736,7 @@ extern "C" int wd_tune_report(wd_ctx *ctx, const wd_field *f, const wd_gauge *Us
     a.U = (const char *)Us->data;
     a.diag = (const float *)diag.p;
     a.nrhs = f->nrhs;
+    a.hc = -0.5f;
     rc = wd_tune_sloppy(ctx, a, f->prec, Us->prec, Us->recon, 0, best_R, best_chunk, ms);
     wd_field_destroy(out);
     return rc;
@@ -759,6 +773,7 @@ extern "C" int wd_apply_M_sloppy(wd_ctx *ctx, wd_field *out, const wd_field *in,
     a.U = (const char *)Us->data;
     a.diag = (const float *)diag.p;
     a.nrhs = n;
+    a.hc = -0.5f;
     a.chunk = site_chunk;
     rc = wd_launch_hop_sloppy(ctx, a, in->prec, Us->prec, Us->recon, rhs_per_thread, dagger, -1,
                               HOP_M);

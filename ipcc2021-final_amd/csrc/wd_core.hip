@@ -122,9 +122,10 @@ extern "C" int wd_create(wd_ctx **out, const int L[4], const int bc[4], int devi
         c->geom.tW = (c->geom.tD > 0 && w8 >= 1 && w8 < c->geom.tD) ? w8 : 0;
     }
     c->xcd_remap = 1;
-    // partials: enough for a both-parity hop launch and the reduction grid
+    // partials: enough for a both-parity hop launch and the streaming
+    // reduction grids (kStreamPartials, 256 KiB)
     const int hop_blocks = (int)(2 * ((Vh + kBlock - 1) / kBlock));
-    c->max_partials = hop_blocks > kRedBlocks ? hop_blocks : kRedBlocks;
+    c->max_partials = hop_blocks > kStreamPartials ? hop_blocks : kStreamPartials;
     c->dscalars_bytes = 1 << 16;
     bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
     ok = ok && hipMalloc(&c->partials, sizeof(double2) * c->max_partials) == hipSuccess;

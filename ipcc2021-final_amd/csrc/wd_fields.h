@@ -48,7 +48,36 @@ __device__ __forceinline__ V ldg(const V *p)
         return *p;
 }
 
+template <bool NT, typename V>
+__device__ __forceinline__ void stg(V *p, V v)
+{
+    if constexpr (NT)
+        __builtin_nontemporal_store(v, p);
+    else
+        *p = v;
+}
+
 // ---- half helpers ------------------------------------------------------------
+typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+
+// Real and imaginary part of a complex packed as half2, widened.  Used as
+// operands of fmaf, these become the f16 sources of v_fma_mix_f32 (op_sel
+// picks the half), so no separate conversion instructions are issued.
+__device__ __forceinline__ float re16(unsigned w) { return (float)__builtin_bit_cast(h2v, w).x; }
+__device__ __forceinline__ float im16(unsigned w) { return (float)__builtin_bit_cast(h2v, w).y; }
+
+// |v|^2 of an F16 site from its packed words and scale: the f16 squares are
+// exact in FP32, the scale is applied once in FP64.
+__device__ __forceinline__ double norm2_packed(const unsigned (&w)[12], float s)
+{
+    float t = 0.f;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+        t = fmaf(re16(w[k]), re16(w[k]), t);
+        t = fmaf(im16(w[k]), im16(w[k]), t);
+    }
+    return (double)t * ((double)s * (double)s);
+}
 __device__ __forceinline__ cpx<float> h2f(unsigned w, float s)
 {
     __half2 h = *reinterpret_cast<__half2 *>(&w);
@@ -70,18 +99,18 @@ struct Spinor;
 template <>
 struct Spinor<WD_F64> {
     using T = double;
-    template <typename R>
+    template <typename R, bool NT = false>
     __device__ static __forceinline__ void load(const void *base, const float *, int stride,
                                                 unsigned i, cpx<R> (&v)[4][3])
     {
         const double2 *b = static_cast<const double2 *>(base);
 #pragma unroll
         for (int k = 0; k < 12; ++k) {
-            const d2v c = *reinterpret_cast<const d2v *>(b + f64_idx(k, i, 12));
+            const d2v c = ldg<NT>(reinterpret_cast<const d2v *>(b + f64_idx(k, i, 12)));
             v[k / 3][k % 3] = {(R)c.x, (R)c.y};
         }
     }
-    template <typename R>
+    template <typename R, bool NT = false>
     __device__ static __forceinline__ void store(void *base, float *, int stride, unsigned i,
                                                  const cpx<R> (&v)[4][3])
     {
@@ -91,7 +120,7 @@ struct Spinor<WD_F64> {
             d2v c;
             c.x = (double)v[k / 3][k % 3].r;
             c.y = (double)v[k / 3][k % 3].i;
-            *reinterpret_cast<d2v *>(b + f64_idx(k, i, 12)) = c;
+            stg<NT>(reinterpret_cast<d2v *>(b + f64_idx(k, i, 12)), c);
         }
     }
 };
@@ -99,19 +128,19 @@ struct Spinor<WD_F64> {
 template <>
 struct Spinor<WD_F32> {
     using T = float;
-    template <typename R>
+    template <typename R, bool NT = false>
     __device__ static __forceinline__ void load(const void *base, const float *, int stride,
                                                 unsigned i, cpx<R> (&v)[4][3])
     {
         const float4 *b = static_cast<const float4 *>(base);
 #pragma unroll
         for (int j = 0; j < 6; ++j) {
-            const f4v c = *reinterpret_cast<const f4v *>(b + (size_t)j * (unsigned)stride + i);
+            const f4v c = ldg<NT>(reinterpret_cast<const f4v *>(b + (size_t)j * (unsigned)stride + i));
             v[(2 * j) / 3][(2 * j) % 3] = {(R)c.x, (R)c.y};
             v[(2 * j + 1) / 3][(2 * j + 1) % 3] = {(R)c.z, (R)c.w};
         }
     }
-    template <typename R>
+    template <typename R, bool NT = false>
     __device__ static __forceinline__ void store(void *base, float *, int stride, unsigned i,
                                                  const cpx<R> (&v)[4][3])
     {
@@ -123,7 +152,7 @@ struct Spinor<WD_F32> {
             c.y = (float)v[(2 * j) / 3][(2 * j) % 3].i;
             c.z = (float)v[(2 * j + 1) / 3][(2 * j + 1) % 3].r;
             c.w = (float)v[(2 * j + 1) / 3][(2 * j + 1) % 3].i;
-            *reinterpret_cast<f4v *>(b + (size_t)j * (unsigned)stride + i) = c;
+            stg<NT>(reinterpret_cast<f4v *>(b + (size_t)j * (unsigned)stride + i), c);
         }
     }
 };
@@ -131,15 +160,15 @@ struct Spinor<WD_F32> {
 template <>
 struct Spinor<WD_F16> {
     using T = float;
-    template <typename R>
+    template <typename R, bool NT = false>
     __device__ static __forceinline__ void load(const void *base, const float *scale, int stride,
                                                 unsigned i, cpx<R> (&v)[4][3])
     {
         const uint4 *b = static_cast<const uint4 *>(base);
-        const float s = scale[i];
+        const float s = ldg<NT>(scale + i);
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-            const u4v c = *reinterpret_cast<const u4v *>(b + (size_t)j * (unsigned)stride + i);
+            const u4v c = ldg<NT>(reinterpret_cast<const u4v *>(b + (size_t)j * (unsigned)stride + i));
             const unsigned w[4] = {c.x, c.y, c.z, c.w};
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
@@ -148,9 +177,19 @@ struct Spinor<WD_F16> {
             }
         }
     }
-    template <typename R>
+    template <typename R, bool NT = false>
     __device__ static __forceinline__ void store(void *base, float *scale, int stride, unsigned i,
                                                  const cpx<R> (&v)[4][3])
+    {
+        unsigned w[12];
+        (void)store_packed<R, NT>(base, scale, stride, i, v, w);
+    }
+    // As store; also returns the packed words and the scale as stored, so a
+    // caller can reduce over the rounded values without reading them back.
+    template <typename R, bool NT = false>
+    __device__ static __forceinline__ float store_packed(void *base, float *scale, int stride,
+                                                         unsigned i, const cpx<R> (&v)[4][3],
+                                                         unsigned (&wout)[12])
     {
         uint4 *b = static_cast<uint4 *>(base);
         float m = 0.f;
@@ -163,22 +202,23 @@ struct Spinor<WD_F16> {
         // propagate, so breakdowns stay visible.
         const bool tiny = m < 1.0e-30f;
         const float inv = tiny ? 0.f : 1.0f / m;
-        scale[i] = tiny ? 0.f : m;
+        const float sc = tiny ? 0.f : m;
+        stg<NT>(scale + i, sc);
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-            unsigned w[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const cpx<R> z = v[(4 * j + q) / 3][(4 * j + q) % 3];
-                w[q] = f2h((float)z.r * inv, (float)z.i * inv);
+                wout[4 * j + q] = f2h((float)z.r * inv, (float)z.i * inv);
             }
             u4v c;
-            c.x = w[0];
-            c.y = w[1];
-            c.z = w[2];
-            c.w = w[3];
-            *reinterpret_cast<u4v *>(b + (size_t)j * (unsigned)stride + i) = c;
+            c.x = wout[4 * j];
+            c.y = wout[4 * j + 1];
+            c.z = wout[4 * j + 2];
+            c.w = wout[4 * j + 3];
+            stg<NT>(reinterpret_cast<u4v *>(b + (size_t)j * (unsigned)stride + i), c);
         }
+        return sc;
     }
 };
 

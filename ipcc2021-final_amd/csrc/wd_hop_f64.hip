@@ -113,6 +113,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 1))) 
                                                     const double2 *__restrict__ in,
                                                     const double2 *__restrict__ U, Geom g,
                                                     int cb_first, int nparity, double a,
+                                                    double hc,
                                                     const double2 *__restrict__ same,
                                                     const double2 *__restrict__ dotv,
                                                     double2 *__restrict__ partials, int remap,
@@ -130,6 +131,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 1))) 
             return;
         if (b.a_rhs)
             a = b.a_rhs[rhs];
+        if (b.c_rhs)
+            hc = b.c_rhs[rhs];
         const size_t ro = (size_t)rhs * b.rhs_elems;
         out += ro;
         in += ro;
@@ -204,7 +207,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 1))) 
             for (int s = 0; s < 4; ++s)
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    res[s][c] = {-0.5 * acc[s][c].r, -0.5 * acc[s][c].i};
+                    res[s][c] = {hc * acc[s][c].r, hc * acc[s][c].i};
                     st(out_p, f64_idx(s * 3 + c, (unsigned)i, 12), res[s][c]);
                 }
         } else {
@@ -214,7 +217,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 1))) 
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     const cpx<double> v = ld(same_p, f64_idx(s * 3 + c, (unsigned)i, 12));
-                    res[s][c] = {fma(a, v.r, -0.5 * acc[s][c].r), fma(a, v.i, -0.5 * acc[s][c].i)};
+                    res[s][c] = {fma(a, v.r, hc * acc[s][c].r), fma(a, v.i, hc * acc[s][c].i)};
                     st(out_p, f64_idx(s * 3 + c, (unsigned)i, 12), res[s][c]);
                 }
         }
@@ -262,7 +265,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 1))) 
 
 template <bool DAG>
 int launch_mode(wd_ctx *ctx, double2 *out, const double2 *in, const double2 *U, int cb, int mode,
-                double a, const double2 *same, const double2 *dotv, double2 *partials,
+                double a, double hc, const double2 *same, const double2 *dotv, double2 *partials,
                 const int *stop, const F64Batch &b, int nrhs)
 {
     const Geom &g = ctx->geom;
@@ -276,15 +279,17 @@ int launch_mode(wd_ctx *ctx, double2 *out, const double2 *in, const double2 *U, 
     switch (mode) {
     case HOP_ONLY:
         hipLaunchKernelGGL((k_hop_f64<DAG, HOP_ONLY>), gd, dim3(kBlock), 0, ctx->stream, out,
-                           in, U, g, cb_first, nparity, a, same, dotv, partials, ctx->xcd_remap, stop, b);
+                           in, U, g, cb_first, nparity, a, hc, same, dotv, partials,
+                           ctx->xcd_remap, stop, b);
         break;
     case HOP_M:
         hipLaunchKernelGGL((k_hop_f64<DAG, HOP_M>), gd, dim3(kBlock), 0, ctx->stream, out,
-                           in, U, g, cb_first, nparity, a, same, dotv, partials, ctx->xcd_remap, stop, b);
+                           in, U, g, cb_first, nparity, a, hc, same, dotv, partials,
+                           ctx->xcd_remap, stop, b);
         break;
     default:
         hipLaunchKernelGGL((k_hop_f64<DAG, HOP_M_DOT>), gd, dim3(kBlock), 0, ctx->stream,
-                           out, in, U, g, cb_first, nparity, a, same, dotv, partials,
+                           out, in, U, g, cb_first, nparity, a, hc, same, dotv, partials,
                            ctx->xcd_remap, stop, b);
         break;
     }
@@ -294,28 +299,50 @@ int launch_mode(wd_ctx *ctx, double2 *out, const double2 *in, const double2 *U, 
 } // namespace
 
 // cb in {0, 1}: one output parity; cb = -1: both parities in one launch.
+// out = a same + hc hop(in) (HOP_ONLY: hc hop(in)); hc = -1/2 is the Wilson
+// operator's coefficient.
+int wd_launch_hop_f64c(wd_ctx *ctx, double2 *out, const double2 *in, const double2 *U, int dagger,
+                       int cb, int mode, double a, double hc, const double2 *same,
+                       const double2 *dotv, double2 *partials, const int *stop)
+{
+    const F64Batch one = {0, nullptr, nullptr, nullptr};
+    if (dagger)
+        return launch_mode<true>(ctx, out, in, U, cb, mode, a, hc, same, dotv, partials, stop,
+                                 one, 1);
+    return launch_mode<false>(ctx, out, in, U, cb, mode, a, hc, same, dotv, partials, stop, one,
+                              1);
+}
+
 int wd_launch_hop_f64(wd_ctx *ctx, double2 *out, const double2 *in, const double2 *U, int dagger,
                       int cb, int mode, double a, const double2 *same, const double2 *dotv,
                       double2 *partials, const int *stop)
 {
-    const F64Batch one = {0, nullptr, nullptr};
-    if (dagger)
-        return launch_mode<true>(ctx, out, in, U, cb, mode, a, same, dotv, partials, stop, one, 1);
-    return launch_mode<false>(ctx, out, in, U, cb, mode, a, same, dotv, partials, stop, one, 1);
+    return wd_launch_hop_f64c(ctx, out, in, U, dagger, cb, mode, a, -0.5, same, dotv, partials,
+                              stop);
 }
 
-// nrhs systems at once (blockIdx.y = rhs); b.mask skips systems, b.a_rhs gives
-// per-system diagonal coefficients; partials are [rhs][workgroup].
+// nrhs systems at once (blockIdx.y = rhs); b.mask skips systems, b.a_rhs and
+// b.c_rhs give per-system diagonal and hop coefficients (else hc);
+// partials are [rhs][workgroup].
+int wd_launch_hop_f64_batch_c(wd_ctx *ctx, double2 *out, const double2 *in, const double2 *U,
+                              int dagger, int cb, int mode, double hc, const double2 *same,
+                              const double2 *dotv, double2 *partials, const int *stop,
+                              const F64Batch &b, int nrhs)
+{
+    if (dagger)
+        return launch_mode<true>(ctx, out, in, U, cb, mode, 0.0, hc, same, dotv, partials, stop,
+                                 b, nrhs);
+    return launch_mode<false>(ctx, out, in, U, cb, mode, 0.0, hc, same, dotv, partials, stop, b,
+                              nrhs);
+}
+
 int wd_launch_hop_f64_batch(wd_ctx *ctx, double2 *out, const double2 *in, const double2 *U,
                             int dagger, int cb, int mode, const double2 *same,
                             const double2 *dotv, double2 *partials, const int *stop,
                             const F64Batch &b, int nrhs)
 {
-    if (dagger)
-        return launch_mode<true>(ctx, out, in, U, cb, mode, 0.0, same, dotv, partials, stop, b,
-                                 nrhs);
-    return launch_mode<false>(ctx, out, in, U, cb, mode, 0.0, same, dotv, partials, stop, b,
-                              nrhs);
+    return wd_launch_hop_f64_batch_c(ctx, out, in, U, dagger, cb, mode, -0.5, same, dotv,
+                                     partials, stop, b, nrhs);
 }
 
 // Number of workgroups a hop launch over `nparity` parities uses (partials count).

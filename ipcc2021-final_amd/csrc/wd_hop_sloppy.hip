@@ -35,14 +35,6 @@ using namespace wdf;
 
 namespace {
 
-typedef _Float16 h2v __attribute__((ext_vector_type(2)));
-
-// Real and imaginary part of a complex packed as half2, widened.  Used as
-// operands of fmaf, these become the f16 sources of v_fma_mix_f32 (op_sel
-// picks the half), so no separate conversion instructions are issued.
-__device__ __forceinline__ float re16(unsigned w) { return (float)__builtin_bit_cast(h2v, w).x; }
-__device__ __forceinline__ float im16(unsigned w) { return (float)__builtin_bit_cast(h2v, w).y; }
-
 // Byte offset of 16-byte chunk q of site j in a [chunks][stride] array; the
 // launcher checks that every field fits 32-bit offsets, so loads use a
 // uniform 64-bit base plus a 32-bit lane offset.
@@ -364,42 +356,43 @@ __global__ __launch_bounds__(kBlock) void k_hop_sloppy(SloppyArgs a, Geom g, int
         const int t = rr / g.Lz;
         const int x = 2 * xh + ((p + y + z + t) & 1);
         const int row = i - xh;
-        // -1/2 of the hopping term times the boundary phase of a crossing hop
-        const float hx = -0.5f * (float)g.bc[0], hy = -0.5f * (float)g.bc[1],
-                    hz = -0.5f * (float)g.bc[2], ht = -0.5f * (float)g.bc[3];
+        // hop coefficient (-1/2 for M) times the boundary phase of a crossing hop
+        const float h0 = a.hc;
+        const float hx = h0 * (float)g.bc[0], hy = h0 * (float)g.bc[1],
+                    hz = h0 * (float)g.bc[2], ht = h0 * (float)g.bc[3];
         {
             const int xf = (x + 1 == Lx) ? 0 : x + 1, xb = (x == 0) ? Lx - 1 : x - 1;
             dir_rhs<SF, GF, R, 0, true, DAG>(acc, a, act, in_b, isc_b, p, i, row + (xf >> 1),
-                                             x == Lx - 1 ? hx : -0.5f, stride);
+                                             x == Lx - 1 ? hx : h0, stride);
             dir_rhs<SF, GF, R, 0, false, DAG>(acc, a, act, in_b, isc_b, p, i, row + (xb >> 1),
-                                              x == 0 ? hx : -0.5f, stride);
+                                              x == 0 ? hx : h0, stride);
         }
         {
             const int s = g.Lxh;
             const int jf = (y + 1 == g.Ly) ? i - (g.Ly - 1) * s : i + s;
             const int jb = (y == 0) ? i + (g.Ly - 1) * s : i - s;
             dir_rhs<SF, GF, R, 1, true, DAG>(acc, a, act, in_b, isc_b, p, i, jf,
-                                             y == g.Ly - 1 ? hy : -0.5f, stride);
+                                             y == g.Ly - 1 ? hy : h0, stride);
             dir_rhs<SF, GF, R, 1, false, DAG>(acc, a, act, in_b, isc_b, p, i, jb,
-                                              y == 0 ? hy : -0.5f, stride);
+                                              y == 0 ? hy : h0, stride);
         }
         {
             const int s = g.Lxh * g.Ly;
             const int jf = (z + 1 == g.Lz) ? i - (g.Lz - 1) * s : i + s;
             const int jb = (z == 0) ? i + (g.Lz - 1) * s : i - s;
             dir_rhs<SF, GF, R, 2, true, DAG>(acc, a, act, in_b, isc_b, p, i, jf,
-                                             z == g.Lz - 1 ? hz : -0.5f, stride);
+                                             z == g.Lz - 1 ? hz : h0, stride);
             dir_rhs<SF, GF, R, 2, false, DAG>(acc, a, act, in_b, isc_b, p, i, jb,
-                                              z == 0 ? hz : -0.5f, stride);
+                                              z == 0 ? hz : h0, stride);
         }
         {
             const int s = g.Lxh * g.Ly * g.Lz;
             const int jf = (t + 1 == g.Lt) ? i - (g.Lt - 1) * s : i + s;
             const int jb = (t == 0) ? i + (g.Lt - 1) * s : i - s;
             dir_rhs<SF, GF, R, 3, true, DAG>(acc, a, act, in_b, isc_b, p, i, jf,
-                                             t == g.Lt - 1 ? ht : -0.5f, stride);
+                                             t == g.Lt - 1 ? ht : h0, stride);
             dir_rhs<SF, GF, R, 3, false, DAG>(acc, a, act, in_b, isc_b, p, i, jb,
-                                              t == 0 ? ht : -0.5f, stride);
+                                              t == 0 ? ht : h0, stride);
         }
     }
 
@@ -553,6 +546,7 @@ int wd_launch_hop_sloppy(wd_ctx *ctx, SloppyArgs a, int spinor_prec, int gauge_p
     WD_ARG_CHECK(ctx, a.chunk >= 0, "sloppy hop: site chunk must be >= 0");
     // every per-parity array (at most 6 x 16 bytes per site) must fit 32-bit offsets
     WD_ARG_CHECK(ctx, 96.0 * g.stride < 4294967296.0, "sloppy hop: lattice too large");
+    WD_ARG_CHECK(ctx, a.hc != 0.f, "sloppy hop: hop coefficient not set");
     a.one = 1.0f;
     WD_ARG_CHECK(ctx, wd_sloppy_supported(spinor_prec, gauge_prec, recon),
                  "sloppy hop: unsupported precision combination");

@@ -100,6 +100,7 @@ int wd_check_launch(wd_ctx *ctx, const char *what);
 // ---- launch helpers ----------------------------------------------------------
 constexpr int kBlock = 256;          // threads per workgroup for site kernels
 constexpr int kRedBlocks = 1024;     // workgroups of the BLAS-1 / reduction kernels
+constexpr int kStreamPartials = 16384; // largest grid of the fused streaming reductions
 constexpr int kNumXCD = 8;           // MI355X: 8 accelerator complex dies
 
 // Hardware hands workgroup b to XCD (b mod 8).  With remap on, logical block
@@ -156,16 +157,26 @@ int wd_launch_gauge_from_ref(wd_ctx *ctx, wd_gauge *U, int mu, const double2 *sr
 int wd_convert_rhs(wd_ctx *ctx, wd_field *dst, int rd, const wd_field *src, int rs);
 
 // batch description for the FP64 kernel: rhs r = blockIdx.y at element offset
-// r * rhs_elems; mask[r] == 0 skips it; a_rhs[r] replaces the diagonal coefficient
+// r * rhs_elems; mask[r] == 0 skips it; a_rhs[r] / c_rhs[r] (when not null)
+// replace the diagonal and the hop coefficient
 struct F64Batch {
     size_t rhs_elems;
     const int *mask;
     const double *a_rhs;
+    const double *c_rhs;
 };
 int wd_launch_hop_f64_batch(wd_ctx *ctx, double2 *out, const double2 *in, const double2 *U,
                             int dagger, int cb, int mode, const double2 *same,
                             const double2 *dotv, double2 *partials, const int *stop,
                             const F64Batch &b, int nrhs);
+// general hop coefficient: out = a same + hc hop(in), hc = -1/2 for the Wilson M
+int wd_launch_hop_f64c(wd_ctx *ctx, double2 *out, const double2 *in, const double2 *U, int dagger,
+                       int cb, int mode, double a, double hc, const double2 *same,
+                       const double2 *dotv, double2 *partials, const int *stop);
+int wd_launch_hop_f64_batch_c(wd_ctx *ctx, double2 *out, const double2 *in, const double2 *U,
+                              int dagger, int cb, int mode, double hc, const double2 *same,
+                              const double2 *dotv, double2 *partials, const int *stop,
+                              const F64Batch &b, int nrhs);
 
 // hop modes for the fused FP64 kernel
 enum HopMode {

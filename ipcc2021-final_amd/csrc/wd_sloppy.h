@@ -20,6 +20,7 @@ struct SloppyArgs {
     double2 *partials;       // [nrhs][nsb] for HOP_M_DOT
     const int *stop;         // whole launch becomes a no-op when *stop != 0
     int nrhs;
+    float hc;                // hop coefficient: out = diag in + hc hop(in); -1/2 for M
     int chunk;               // site blocks per L2 chunk (0: groups of one block consecutive)
     float one;               // 1.0f, set by the launcher (an operand the compiler cannot fold)
     // filled by the launcher

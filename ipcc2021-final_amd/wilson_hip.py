@@ -44,7 +44,7 @@ EXPORTED = [
     "wd_field_zero", "wd_field_copy",
     "wd_hop", "wd_diag", "wd_apply_M", "wd_apply_MdagM",
     "wd_norm2", "wd_dot",
-    "wd_cg_params_default", "wd_cg", "wd_time_op", "wd_set_option",
+    "wd_cg_params_default", "wd_cg", "wd_cg_eo", "wd_time_op", "wd_set_option",
     "wd_mixed_params_default", "wd_cg_mixed", "wd_apply_M_sloppy", "wd_tune_report",
     "wdg_gauge_random_su3", "wdg_gauge_near_identity", "wdg_gauge_unit",
     "wdg_gauge_flip_boundary", "wdg_spinor_gaussian", "wdg_spinor_point", "wdg_philox4x32",
@@ -145,6 +145,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "wd_dot": ([vp, vp, vp, dp], i),
         "wd_cg_params_default": ([ctypes.POINTER(CGParams)], None),
         "wd_cg": ([vp, vp, vp, vp, d, ctypes.POINTER(CGParams), ctypes.POINTER(CGReport)], i),
+        "wd_cg_eo": ([vp, vp, vp, vp, d, ctypes.POINTER(CGParams), ctypes.POINTER(CGReport)], i),
         "wd_mixed_params_default": ([ctypes.POINTER(MixedParams)], None),
         "wd_cg_mixed": ([vp, vp, vp, vp, vp, dp, ctypes.POINTER(MixedParams),
                          ctypes.POINTER(MixedReport), ctypes.POINTER(d)], i),
@@ -378,7 +379,9 @@ class Context:
 
     def cg(self, x, b, U, mass: float, max_iter: int = 500, tol: float = 1e-9,
            tol_relative: bool = False, use_x0: bool = True, check_every: int = 8,
-           use_graph: bool = True, history: bool = True):
+           use_graph: bool = True, history: bool = True, even_odd: bool = False):
+        """CGNR on the full M^dag M (wd_cg, the reference's algorithm) or, with
+        even_odd, on the Schur complement of the even sites (wd_cg_eo)."""
         p = CGParams()
         self.lib.wd_cg_params_default(ctypes.byref(p))
         p.max_iter, p.tol, p.tol_relative = max_iter, tol, int(tol_relative)
@@ -387,8 +390,9 @@ class Context:
         if history:
             p.rsd_history = hist.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
         rep = CGReport()
-        rc = self.lib.wd_cg(self.h, x.h, b.h, U.h, mass, ctypes.byref(p), ctypes.byref(rep))
-        self.check(rc, "wd_cg")
+        fn, name = ((self.lib.wd_cg_eo, "wd_cg_eo") if even_odd else (self.lib.wd_cg, "wd_cg"))
+        rc = fn(self.h, x.h, b.h, U.h, mass, ctypes.byref(p), ctypes.byref(rep))
+        self.check(rc, name)
         return {
             "status": rc,
             "iterations": rep.iterations,

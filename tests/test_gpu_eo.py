@@ -1,0 +1,113 @@
+"""Schur even-odd preconditioned CGNR (wd_cg_eo) against the oracle.
+
+A new algorithm beside the reference's full-lattice CG, so the parity target
+is the solution, not the iteration count: the solution must satisfy the
+reference's stop test (FP64 |M^dag b - M^dag M x| < tol |M^dag b|, recomputed
+here with the oracle's M, which is pinned to the compiled reference), agree
+with wd_cg's solution, and take fewer iterations.  Closed forms: all-zero
+links make M = (4+m) 1, so x = b / (4+m) after one iteration; b = 0 gives x = 0.
+"""
+import numpy as np
+import pytest
+
+from oracle_py import kappa_to_mass, libc_rand_x0, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _normal_residual(oracle, x, b, A, L, mass):
+    Mdb = oracle.dslash(b, A, L, mass, 1)
+    r = Mdb - oracle.dslash(oracle.dslash(x, A, L, mass, 0), A, L, mass, 1)
+    return np.linalg.norm(r) / np.linalg.norm(Mdb)
+
+
+@pytest.mark.parametrize("kind,kappa,bc", [("su3", 0.12, [1, 1, 1, -1]),
+                                            ("near", 0.135, [1, 1, 1, -1]),
+                                            ("su3", 0.10, [1, 1, 1, 1])])
+def test_cg_eo_solution(w, oracle, kind, kappa, bc):
+    L = [8, 8, 8, 8]
+    A = w.gauge_random_su3(L, 31) if kind == "su3" else w.gauge_near_identity(L, 31, 0.3)
+    b = w.spinor_gaussian(L, 32)
+    mass = kappa_to_mass(kappa)
+    print(f"cg_eo {kind} kappa={kappa} bc={bc}", flush=True)
+    with w.Context(L, bc=bc) as ctx:
+        U = ctx.gauge(A)
+        fb = ctx.field(data=b)
+        xe, xf = ctx.field(), ctx.field()
+        eo = ctx.cg(xe, fb, U, mass, max_iter=5000, tol=1e-10, tol_relative=True, use_x0=False,
+                    even_odd=True)
+        full = ctx.cg(xf, fb, U, mass, max_iter=5000, tol=1e-10, tol_relative=True,
+                      use_x0=False)
+        x_eo, x_full = xe.download(), xf.download()
+    print(f"iterations eo={eo['iterations']} full={full['iterations']} "
+          f"true_rsd={eo['true_rsd'] / eo['rhs_norm']:.3e}", flush=True)
+    assert eo["converged"] and full["converged"]
+    assert eo["true_rsd"] <= 1e-10 * eo["rhs_norm"]
+    assert eo["iterations"] < full["iterations"]
+    Af = [a.copy() for a in A]
+    if bc[3] == -1:
+        w.gauge_flip_boundary(L, 3, Af)
+    assert _normal_residual(oracle, x_eo, b, Af, L, mass) < 1.05e-10
+    assert rel_l2(x_eo, x_full) < 1e-7
+
+
+def test_cg_eo_start_vector_and_history(w, oracle):
+    L = [4, 4, 4, 8]
+    A = w.gauge_near_identity(L, 41, 0.3)
+    b = w.spinor_gaussian(L, 42)
+    x0 = libc_rand_x0(77, w.volume(L) * 12)
+    mass = 0.05
+    with w.Context(L) as ctx:
+        U = ctx.gauge(A)
+        fb, x = ctx.field(data=b), ctx.field(data=x0)
+        res = ctx.cg(x, fb, U, mass, max_iter=2000, tol=1e-11, tol_relative=True, use_x0=True,
+                     even_odd=True, check_every=3)
+        xs = x.download()
+    assert res["converged"]
+    assert _normal_residual(oracle, xs, b, A, L, mass) < 1.05e-11
+    h = res["history"]
+    assert len(h) == res["iterations"] and np.all(np.isfinite(h)) and h[-1] < h[0]
+
+
+def test_cg_eo_zero_links_closed_form(w):
+    L = [4, 4, 4, 4]
+    A = [np.zeros(w.volume(L) * 9, dtype=np.complex128) for _ in range(4)]
+    b = w.spinor_gaussian(L, 5)
+    mass = 0.3
+    with w.Context(L) as ctx:
+        U = ctx.gauge(A)
+        fb, x = ctx.field(data=b), ctx.field()
+        res = ctx.cg(x, fb, U, mass, max_iter=10, tol=1e-12, tol_relative=True, use_x0=False,
+                     even_odd=True)
+        xs = x.download()
+    assert res["converged"] and res["iterations"] == 1
+    assert rel_l2(xs, b / (4 + mass)) < 1e-14
+
+
+def test_cg_eo_zero_source(w):
+    L = [4, 4, 4, 4]
+    with w.Context(L) as ctx:
+        U = ctx.gauge(w.gauge_random_su3(L, 3))
+        fb = ctx.field(data=np.zeros(w.volume(L) * 12, dtype=np.complex128))
+        x = ctx.field(data=w.spinor_gaussian(L, 4))
+        res = ctx.cg(x, fb, U, 0.1, tol=1e-10, tol_relative=True, use_x0=False, even_odd=True)
+        assert res["converged"] and res["iterations"] == 0
+        assert np.all(x.download() == 0)
+
+
+def test_cg_eo_matches_graph_and_eager(w):
+    L = [8, 8, 8, 8]
+    A = w.gauge_random_su3(L, 51)
+    b = w.spinor_gaussian(L, 52)
+    out = {}
+    with w.Context(L, bc=[1, 1, 1, -1]) as ctx:
+        U = ctx.gauge(A)
+        fb = ctx.field(data=b)
+        for g in (True, False):
+            x = ctx.field()
+            r = ctx.cg(x, fb, U, kappa_to_mass(0.12), max_iter=3000, tol=1e-10,
+                       tol_relative=True, use_x0=False, even_odd=True, use_graph=g, check_every=7)
+            out[g] = (r["iterations"], x.download(), r["history"])
+    assert out[True][0] == out[False][0]
+    assert np.array_equal(out[True][1], out[False][1])
+    assert np.array_equal(out[True][2], out[False][2])
