@@ -179,20 +179,24 @@ typedef struct wd_mixed_params {
                            operator; 0 = off, -1 = autotune (with rhs_per_thread) */
     int check_every;    /* iterations between host polls of the convergence flags */
     int use_graph;      /* replay each block of check_every iterations as one hipGraph */
+    int even_odd;       /* 1: iterate on the Schur complement of the even sites (as
+                           wd_cg_eo; x_o reconstructed, tol checked on the full system
+                           with the FP64 operator), 0: on the full M^dag M */
 } wd_mixed_params;
 
 typedef struct wd_mixed_report {
     int iterations;       /* inner iterations */
     int reliable_updates; /* FP64 true-residual recomputations */
     int converged;        /* 1 when the true residual met tol */
-    double true_rel_rsd;  /* |M^dag b - M^dag M x| / |M^dag b| at the last reliable update */
+    double true_rel_rsd;  /* |M^dag b - M^dag M x| / |M^dag b| at the last reliable update
+                             (even_odd: at the last check of the full system) */
     double rhs_norm;      /* |M^dag b| */
     int rhs_per_thread;   /* the rhs-per-thread the operator ran with */
     int site_chunk;       /* the L2 chunk the operator ran with */
 } wd_mixed_report;
 
 /* max_iter 100000, tol 1e-10, delta 1e-2, F16 spinors, autotuned rhs per
- * thread and site chunk, check_every 16, graphs on. */
+ * thread and site chunk, check_every 16, graphs on, full system (even_odd 0). */
 void wd_mixed_params_default(wd_mixed_params *p);
 
 /* Solves M_r^dag M_r x_r = M_r^dag b_r for all nrhs systems of b at once

@@ -29,6 +29,7 @@
 //   alldone stop flag when no system is active
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <map>
@@ -51,7 +52,9 @@ struct MxState {
     double target;   // tol * |M^dag b|
     double rsd;      // last sloppy |r_s|
     double true_rsd; // |M^dag b - M^dag M x| at the last reliable update
-    double b_norm;   // |M^dag b|
+    double b_norm;   // |M^dag b| (even-odd: of the Schur system, |A^dag bh|)
+    double full_b;   // even-odd: |M^dag b| of the full system
+    double full_rsd; // even-odd: |M^dag b - M^dag M x| at the last full check
     int iter, status, updates, pad; // status: 0 running, 1 converged, 2 breakdown, 3 limit
 };
 
@@ -80,15 +83,14 @@ __global__ __launch_bounds__(kBlock) void k_mx_xpay(char *__restrict__ p, float 
     const size_t so = ((size_t)rhs * 2 + par) * stride;
     const float beta = (float)st[rhs].beta;
     cpx<float> vp[4][3], vr[4][3];
-    // streamed once per iteration: non-temporal
-    Spinor<SF>::template load<float, true>(p + off, ps ? ps + so : nullptr, stride, (unsigned)i, vp);
-    Spinor<SF>::template load<float, true>(r + off, rs ? rs + so : nullptr, stride, (unsigned)i, vr);
+    Spinor<SF>::template load<float, false>(p + off, ps ? ps + so : nullptr, stride, (unsigned)i, vp);
+    Spinor<SF>::template load<float, false>(r + off, rs ? rs + so : nullptr, stride, (unsigned)i, vr);
 #pragma unroll
     for (int s = 0; s < 4; ++s)
 #pragma unroll
         for (int c = 0; c < 3; ++c)
             vp[s][c] = {fmaf(beta, vp[s][c].r, vr[s][c].r), fmaf(beta, vp[s][c].i, vr[s][c].i)};
-    Spinor<SF>::template store<float, true>(p + off, ps ? ps + so : nullptr, stride, (unsigned)i,
+    Spinor<SF>::template store<float, false>(p + off, ps ? ps + so : nullptr, stride, (unsigned)i,
                                             vp);
 }
 
@@ -111,13 +113,12 @@ __global__ __launch_bounds__(kBlock) void k_mx_update(
         const size_t so = ((size_t)rhs * 2 + par) * stride;
         const float alpha = (float)st[rhs].alpha;
         cpx<float> vx[4][3], vr[4][3], vp[4][3], vq[4][3];
-        // streamed once per iteration: non-temporal
-        Spinor<WD_F32>::template load<float, true>(xs + xoff, nullptr, stride, (unsigned)i, vx);
-        Spinor<SF>::template load<float, true>(r + off, rs ? rs + so : nullptr, stride,
+        Spinor<WD_F32>::template load<float, false>(xs + xoff, nullptr, stride, (unsigned)i, vx);
+        Spinor<SF>::template load<float, false>(r + off, rs ? rs + so : nullptr, stride,
                                                (unsigned)i, vr);
-        Spinor<SF>::template load<float, true>(p + off, ps ? ps + so : nullptr, stride,
+        Spinor<SF>::template load<float, false>(p + off, ps ? ps + so : nullptr, stride,
                                                (unsigned)i, vp);
-        Spinor<SF>::template load<float, true>(q + off, qs ? qs + so : nullptr, stride,
+        Spinor<SF>::template load<float, false>(q + off, qs ? qs + so : nullptr, stride,
                                                (unsigned)i, vq);
 #pragma unroll
         for (int s = 0; s < 4; ++s)
@@ -128,15 +129,15 @@ __global__ __launch_bounds__(kBlock) void k_mx_update(
                 vr[s][c] = {fmaf(-alpha, vq[s][c].r, vr[s][c].r),
                             fmaf(-alpha, vq[s][c].i, vr[s][c].i)};
             }
-        Spinor<WD_F32>::template store<float, true>(xs + xoff, nullptr, stride, (unsigned)i, vx);
+        Spinor<WD_F32>::template store<float, false>(xs + xoff, nullptr, stride, (unsigned)i, vx);
         // |r_s|^2 of the stored (rounded) residual
         if constexpr (SF == WD_F16) {
             unsigned w[12];
-            const float sc = Spinor<WD_F16>::template store_packed<float, true>(
+            const float sc = Spinor<WD_F16>::template store_packed<float, false>(
                 r + off, rs + so, stride, (unsigned)i, vr, w);
             acc = norm2_packed(w, sc);
         } else {
-            Spinor<SF>::template store<float, true>(r + off, nullptr, stride, (unsigned)i, vr);
+            Spinor<SF>::template store<float, false>(r + off, nullptr, stride, (unsigned)i, vr);
 #pragma unroll
             for (int s = 0; s < 4; ++s)
 #pragma unroll
@@ -321,6 +322,89 @@ __global__ __launch_bounds__(1024) void k_mx_relfin(const double2 *__restrict__ 
     }
 }
 
+// ---- even-odd: checks on the full system ------------------------------------------
+// partial |bp - q|^2 (|bp|^2 when q == nullptr) over both parities, systems
+// with mask[r] (all when mask == nullptr)
+__global__ __launch_bounds__(kBlock) void k_mx_fullres(const double2 *__restrict__ bp,
+                                                       const double2 *__restrict__ q,
+                                                       size_t x_rhs_elems, int stride,
+                                                       const int *__restrict__ mask,
+                                                       double2 *__restrict__ partials)
+{
+    const int rhs = blockIdx.y;
+    if (mask && !mask[rhs])
+        return;
+    const int e = blockIdx.x * kBlock + threadIdx.x;
+    double acc = 0.0;
+    if (e < 2 * stride) {
+        const int par = e / stride, i = e - par * stride;
+        const size_t xo = rhs * x_rhs_elems + (size_t)par * 12 * stride;
+        cpx<double> vb[4][3], vq[4][3];
+        Spinor<WD_F64>::load(bp + xo, nullptr, stride, (unsigned)i, vb);
+        if (q)
+            Spinor<WD_F64>::load(q + xo, nullptr, stride, (unsigned)i, vq);
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double dr = q ? vb[s][c].r - vq[s][c].r : vb[s][c].r;
+                const double di = q ? vb[s][c].i - vq[s][c].i : vb[s][c].i;
+                acc = fma(dr, dr, acc);
+                acc = fma(di, di, acc);
+            }
+    }
+    block_reduce_store(acc, 0.0, partials + (size_t)rhs * gridDim.x + blockIdx.x);
+}
+
+// full_b = |M^dag b| per system
+__global__ __launch_bounds__(1024) void k_mx_fullb(const double2 *__restrict__ partials, int count,
+                                                   MxState *__restrict__ st)
+{
+    const double2 s = sum_rhs_partials(partials, count);
+    if (threadIdx.x == 0)
+        st[blockIdx.x].full_b = sqrt(s.x);
+}
+
+// Systems that met the Schur target (check[r]): accept when the full-system
+// residual meets tol, else resume them with a Schur target tightened by the
+// observed ratio (the state is that of a reliable update: r_s is the true
+// Schur residual, x_s = 0).
+__global__ __launch_bounds__(1024) void k_mx_fullfin(const double2 *__restrict__ partials,
+                                                     int count, MxState *__restrict__ st,
+                                                     int *__restrict__ active,
+                                                     int *__restrict__ check, double tol,
+                                                     int max_iter)
+{
+    const int r = blockIdx.x;
+    if (!check[r])
+        return;
+    const double2 s = sum_rhs_partials(partials, count);
+    if (threadIdx.x == 0) {
+        MxState &m = st[r];
+        const double full = sqrt(s.x);
+        m.full_rsd = full;
+        check[r] = 0;
+        if (full <= tol * m.full_b || !(m.full_b > 0.0) || m.iter >= max_iter ||
+            !isfinite(full))
+            return;
+        const double rr = m.true_rsd * m.true_rsd;
+        m.target *= 0.5 * tol * m.full_b / full;
+        m.beta = rr / m.rho;
+        m.rho = rr;
+        m.maxr = m.true_rsd;
+        m.status = 0;
+        active[r] = 1;
+    }
+}
+
+// check[r] = 1 for the systems the Schur iteration declared converged
+__global__ void k_mx_markconv(const MxState *__restrict__ st, int nrhs, int *__restrict__ check)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nrhs)
+        check[r] = st[r].status == 1 ? 1 : 0;
+}
+
 __global__ void k_mx_alldone(const int *__restrict__ active, int nrhs, int *__restrict__ stop)
 {
     if (threadIdx.x == 0) {
@@ -437,6 +521,7 @@ extern "C" void wd_mixed_params_default(wd_mixed_params *p)
     p->site_chunk = -1;
     p->check_every = 16;
     p->use_graph = 1;
+    p->even_odd = 0;
 }
 
 extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd_gauge *U,
@@ -468,35 +553,49 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
     const int stride = (int)ctx->stride;
     int rc;
 
-    // device work vectors
-    FieldBuf Mdb, t64, q64, rs, p, q, qq, xs;
+    const bool eo = prm->even_odd != 0;
+    // device work vectors (even-odd: the Schur system lives on the even half;
+    // the odd half of an operator's output buffer holds its intermediate hop)
+    FieldBuf Mdb, Mfull, t64, q64, rs, p, q, qq, xs;
     if ((rc = wd_field_create(ctx, &Mdb.f, WD_F64, n)) || (rc = wd_field_create(ctx, &t64.f, WD_F64, n)) ||
         (rc = wd_field_create(ctx, &q64.f, WD_F64, n)) || (rc = wd_field_create(ctx, &rs.f, sp, n)) ||
         (rc = wd_field_create(ctx, &p.f, sp, n)) || (rc = wd_field_create(ctx, &q.f, sp, n)) ||
-        (rc = wd_field_create(ctx, &qq.f, sp, n)) || (rc = wd_field_create(ctx, &xs.f, WD_F32, n)))
+        (rc = wd_field_create(ctx, &qq.f, sp, n)) || (rc = wd_field_create(ctx, &xs.f, WD_F32, n)) ||
+        (eo && (rc = wd_field_create(ctx, &Mfull.f, WD_F64, n))))
         return rc;
-    const int eg = (2 * stride + kBlock - 1) / kBlock; // elementwise workgroups per rhs
-    const int hop_nsb = 2 * (int)((ctx->Vh + kBlock - 1) / kBlock);
-    const int nsb_max = eg > hop_nsb ? eg : hop_nsb;
-    DevBuf partials, state, flags, diag_d, diag_f;
+    const int eg_full = (2 * stride + kBlock - 1) / kBlock; // elementwise workgroups per rhs
+    const int eg = eo ? (stride + kBlock - 1) / kBlock : eg_full; // (even half only)
+    const int bpp = (int)((ctx->Vh + kBlock - 1) / kBlock);
+    const int hop_nsb = eo ? bpp : 2 * bpp; // partials of the (last) operator launch
+    const int nsb_max = std::max(eg_full, 2 * bpp);
+    DevBuf partials, state, flags, diag_d, diag_f, coef;
     if ((rc = partials.alloc(ctx, sizeof(double2) * (size_t)n * nsb_max)) ||
         (rc = state.alloc(ctx, sizeof(MxState) * n)) ||
-        (rc = flags.alloc(ctx, sizeof(int) * (2 * (size_t)n + 4))) ||
-        (rc = diag_d.alloc(ctx, sizeof(double) * n)) || (rc = diag_f.alloc(ctx, sizeof(float) * n)))
+        (rc = flags.alloc(ctx, sizeof(int) * (3 * (size_t)n + 4))) ||
+        (rc = diag_d.alloc(ctx, sizeof(double) * n)) || (rc = diag_f.alloc(ctx, sizeof(float) * n)) ||
+        (rc = coef.alloc(ctx, sizeof(double) * 3 * n)))
         return rc;
     MxState *st = (MxState *)state.p;
-    int *active = (int *)flags.p, *reliable = active + n, *stop = reliable + n;
+    int *active = (int *)flags.p, *reliable = active + n, *check = reliable + n, *stop = check + n;
     double2 *part = (double2 *)partials.p;
+    // per-system coefficients: a_r = 4 + m_r; even-odd also a_r^2 (diagonal of
+    // A), 1/a_r and 1/(2 a_r) (reconstruction of x_o)
+    double *a2_d = (double *)coef.p, *inva_d = a2_d + n, *inv2a_d = inva_d + n;
     {
-        std::vector<double> ad(n);
+        std::vector<double> ad(n), cd(3 * (size_t)n);
         std::vector<float> af(n);
         for (int r = 0; r < n; ++r) {
             ad[r] = 4.0 + masses[r];
-            af[r] = (float)ad[r];
+            af[r] = eo ? (float)(ad[r] * ad[r]) : (float)ad[r];
+            cd[r] = ad[r] * ad[r];
+            cd[n + r] = 1.0 / ad[r];
+            cd[2 * n + r] = 0.5 / ad[r];
         }
         WD_HIP_CHECK(ctx, hipMemcpyAsync(diag_d.p, ad.data(), sizeof(double) * n,
                                          hipMemcpyHostToDevice, ctx->stream));
         WD_HIP_CHECK(ctx, hipMemcpyAsync(diag_f.p, af.data(), sizeof(float) * n,
+                                         hipMemcpyHostToDevice, ctx->stream));
+        WD_HIP_CHECK(ctx, hipMemcpyAsync(coef.p, cd.data(), sizeof(double) * 3 * n,
                                          hipMemcpyHostToDevice, ctx->stream));
         WD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     }
@@ -504,28 +603,65 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
     const size_t xe = x->rhs_elems;
     const size_t srb = rs.f->rhs_elems * prec_bytes_complex(sp);
     const size_t xsb = xs.f->rhs_elems * prec_bytes_complex(WD_F32);
-    F64Batch all = {xe, nullptr, (const double *)diag_d.p, nullptr};
-    F64Batch rel = {xe, reliable, (const double *)diag_d.p, nullptr};
+    const double *ddiag = (const double *)diag_d.p;
+    F64Batch all = {xe, nullptr, ddiag, nullptr};
+    F64Batch rel = {xe, reliable, ddiag, nullptr};
+    F64Batch all2 = {xe, nullptr, a2_d, nullptr};
+    F64Batch rel2 = {xe, reliable, a2_d, nullptr};
+    double2 *xd = (double2 *)x->data;
+    const double2 *bd = (const double2 *)b->data;
+    double2 *Mdbd = (double2 *)Mdb.f->data, *t64d = (double2 *)t64.f->data,
+            *q64d = (double2 *)q64.f->data;
 
-    // Mdb = M^dag b for every system; x = 0; r_s = Mdb; rho = |Mdb|^2
-    if ((rc = wd_launch_hop_f64_batch(ctx, (double2 *)Mdb.f->data, (const double2 *)b->data, Ud, 1,
-                                      -1, HOP_M, (const double2 *)b->data, nullptr, nullptr,
-                                      nullptr, all, n)))
-        return rc;
+    // FP64 A v (dag 0) or A^dag v on the even half, A = a^2 - D_eo D_oe / 4;
+    // the odd half of out holds the intermediate +D_oe v / 2
+    auto apply_A64 = [&](int dag, const double2 *in, double2 *out, const F64Batch &b1,
+                         const F64Batch &b2, const int *stp) -> int {
+        int e = wd_launch_hop_f64_batch_c(ctx, out, in, Ud, dag, 1, HOP_ONLY, 0.5, nullptr,
+                                          nullptr, nullptr, stp, b1, n);
+        if (e)
+            return e;
+        return wd_launch_hop_f64_batch_c(ctx, out, out, Ud, dag, 0, HOP_M, -0.5, in, nullptr,
+                                         nullptr, stp, b2, n);
+    };
+
+    // right-hand side of the CG system: M^dag b, or A^dag bh with
+    // bh = a b_e + D_eo b_o / 2 (even-odd; then also M^dag b for the full check)
+    if (!eo) {
+        if ((rc = wd_launch_hop_f64_batch(ctx, Mdbd, bd, Ud, 1, -1, HOP_M, bd, nullptr, nullptr,
+                                          nullptr, all, n)))
+            return rc;
+    } else {
+        if ((rc = wd_launch_hop_f64_batch_c(ctx, t64d, bd, Ud, 0, 0, HOP_M, 0.5, bd, nullptr,
+                                            nullptr, nullptr, all, n)) ||
+            (rc = apply_A64(1, t64d, Mdbd, all, all2, nullptr)) ||
+            (rc = wd_launch_hop_f64_batch(ctx, (double2 *)Mfull.f->data, bd, Ud, 1, -1, HOP_M,
+                                          bd, nullptr, nullptr, nullptr, all, n)))
+            return rc;
+    }
     if ((rc = wd_launch_zero(ctx, x->data, x->bytes)))
         return rc;
+    // r_s = rhs (even half for even-odd), rho = |rhs|^2
     if (sp == WD_F16)
         hipLaunchKernelGGL(k_mx_relr<WD_F16>, dim3(eg, n), dim3(kBlock), 0, ctx->stream,
-                           (char *)rs.f->data, rs.f->scale, (const double2 *)Mdb.f->data,
+                           (char *)rs.f->data, rs.f->scale, (const double2 *)Mdbd,
                            (const double2 *)nullptr, xe, srb, stride, (const int *)nullptr, part);
     else
         hipLaunchKernelGGL(k_mx_relr<WD_F32>, dim3(eg, n), dim3(kBlock), 0, ctx->stream,
-                           (char *)rs.f->data, rs.f->scale, (const double2 *)Mdb.f->data,
+                           (char *)rs.f->data, rs.f->scale, (const double2 *)Mdbd,
                            (const double2 *)nullptr, xe, srb, stride, (const int *)nullptr, part);
     if ((rc = wd_check_launch(ctx, "k_mx_relr")))
         return rc;
+    // even-odd: the Schur target starts at half the full-system tolerance
     hipLaunchKernelGGL(k_mx_init, dim3(n), dim3(1024), 0, ctx->stream, part, eg, st, active,
-                       reliable, prm->tol);
+                       reliable, eo ? 0.5 * prm->tol : prm->tol);
+    if (eo) {
+        hipLaunchKernelGGL(k_mx_fullres, dim3(eg_full, n), dim3(kBlock), 0, ctx->stream,
+                           (const double2 *)Mfull.f->data, (const double2 *)nullptr, xe, stride,
+                           (const int *)nullptr, part);
+        hipLaunchKernelGGL(k_mx_fullb, dim3(n), dim3(1024), 0, ctx->stream,
+                           (const double2 *)part, eg_full, st);
+    }
     hipLaunchKernelGGL(k_mx_alldone, dim3(1), dim3(64), 0, ctx->stream, active, n, stop);
     if ((rc = wd_check_launch(ctx, "k_mx_init")))
         return rc;
@@ -558,6 +694,31 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
     A.chunk = C;
     const int nsb = hop_nsb;
 
+    // sloppy out = M in (full) or A in (even-odd), with the partials of
+    // (dotv, out) for HOP_M_DOT
+    auto sloppy_op = [&](int dag, wd_field *out, const wd_field *in, int mode,
+                         const wd_field *dotv) -> int {
+        SloppyArgs m = A;
+        m.out = (char *)out->data;
+        m.out_scale = out->scale;
+        m.in = m.same = (const char *)in->data;
+        m.in_scale = m.same_scale = in->scale;
+        if (dotv) {
+            m.dotv = (const char *)dotv->data;
+            m.dotv_scale = dotv->scale;
+        }
+        if (!eo)
+            return wd_launch_hop_sloppy(ctx, m, sp, Us->prec, Us->recon, R, dag, -1, mode);
+        SloppyArgs h = m; // +D_oe in / 2 into the odd half of out
+        h.hc = 0.5f;
+        int e = wd_launch_hop_sloppy(ctx, h, sp, Us->prec, Us->recon, R, dag, 1, HOP_ONLY);
+        if (e)
+            return e;
+        m.in = (const char *)out->data; // a^2 in_e - D_eo (odd half of out) / 2
+        m.in_scale = out->scale;
+        return wd_launch_hop_sloppy(ctx, m, sp, Us->prec, Us->recon, R, dag, 0, mode);
+    };
+
     auto iteration = [&]() -> int {
         int e;
         if (sp == WD_F16)
@@ -572,21 +733,8 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
                                (const int *)active);
         if ((e = wd_check_launch(ctx, "k_mx_xpay")))
             return e;
-        SloppyArgs m = A;
-        m.out = (char *)qq.f->data;
-        m.out_scale = qq.f->scale;
-        m.in = m.same = (const char *)p.f->data;
-        m.in_scale = m.same_scale = p.f->scale;
-        if ((e = wd_launch_hop_sloppy(ctx, m, sp, Us->prec, Us->recon, R, 0, -1, HOP_M)))
-            return e;
-        SloppyArgs md = A;
-        md.out = (char *)q.f->data;
-        md.out_scale = q.f->scale;
-        md.in = md.same = (const char *)qq.f->data;
-        md.in_scale = md.same_scale = qq.f->scale;
-        md.dotv = (const char *)p.f->data;
-        md.dotv_scale = p.f->scale;
-        if ((e = wd_launch_hop_sloppy(ctx, md, sp, Us->prec, Us->recon, R, 1, -1, HOP_M_DOT)))
+        if ((e = sloppy_op(0, qq.f, p.f, HOP_M, nullptr)) ||
+            (e = sloppy_op(1, q.f, qq.f, HOP_M_DOT, p.f)))
             return e;
         hipLaunchKernelGGL(k_mx_alpha, dim3(n), dim3(1024), 0, ctx->stream,
                            (const double2 *)part, nsb, st, active);
@@ -605,34 +753,63 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
         hipLaunchKernelGGL(k_mx_decide, dim3(n), dim3(1024), 0, ctx->stream,
                            (const double2 *)part, eg, st, (const int *)active, reliable,
                            prm->max_iter, prm->delta);
-        hipLaunchKernelGGL(k_mx_relx, dim3(eg, n), dim3(kBlock), 0, ctx->stream,
-                           (double2 *)x->data, (char *)xs.f->data, xe, xsb, stride,
-                           (const int *)reliable);
+        hipLaunchKernelGGL(k_mx_relx, dim3(eg, n), dim3(kBlock), 0, ctx->stream, xd,
+                           (char *)xs.f->data, xe, xsb, stride, (const int *)reliable);
         if ((e = wd_check_launch(ctx, "k_mx_relx")))
             return e;
-        if ((e = wd_launch_hop_f64_batch(ctx, (double2 *)t64.f->data, (const double2 *)x->data,
-                                         Ud, 0, -1, HOP_M, (const double2 *)x->data, nullptr,
-                                         nullptr, stop, rel, n)) ||
-            (e = wd_launch_hop_f64_batch(ctx, (double2 *)q64.f->data,
-                                         (const double2 *)t64.f->data, Ud, 1, -1, HOP_M,
-                                         (const double2 *)t64.f->data, nullptr, nullptr, stop,
-                                         rel, n)))
-            return e;
+        if (!eo) {
+            if ((e = wd_launch_hop_f64_batch(ctx, t64d, xd, Ud, 0, -1, HOP_M, xd, nullptr,
+                                             nullptr, stop, rel, n)) ||
+                (e = wd_launch_hop_f64_batch(ctx, q64d, t64d, Ud, 1, -1, HOP_M, t64d, nullptr,
+                                             nullptr, stop, rel, n)))
+                return e;
+        } else {
+            if ((e = apply_A64(0, xd, t64d, rel, rel2, stop)) ||
+                (e = apply_A64(1, t64d, q64d, rel, rel2, stop)))
+                return e;
+        }
         if (sp == WD_F16)
             hipLaunchKernelGGL(k_mx_relr<WD_F16>, dim3(eg, n), dim3(kBlock), 0, ctx->stream,
-                               (char *)rs.f->data, rs.f->scale, (const double2 *)Mdb.f->data,
-                               (const double2 *)q64.f->data, xe, srb, stride,
-                               (const int *)reliable, part);
+                               (char *)rs.f->data, rs.f->scale, (const double2 *)Mdbd,
+                               (const double2 *)q64d, xe, srb, stride, (const int *)reliable,
+                               part);
         else
             hipLaunchKernelGGL(k_mx_relr<WD_F32>, dim3(eg, n), dim3(kBlock), 0, ctx->stream,
-                               (char *)rs.f->data, rs.f->scale, (const double2 *)Mdb.f->data,
-                               (const double2 *)q64.f->data, xe, srb, stride,
-                               (const int *)reliable, part);
+                               (char *)rs.f->data, rs.f->scale, (const double2 *)Mdbd,
+                               (const double2 *)q64d, xe, srb, stride, (const int *)reliable,
+                               part);
         hipLaunchKernelGGL(k_mx_relfin, dim3(n), dim3(1024), 0, ctx->stream,
                            (const double2 *)part, eg, st, active, reliable, prm->max_iter);
         hipLaunchKernelGGL(k_mx_alldone, dim3(1), dim3(64), 0, ctx->stream,
                            (const int *)active, n, stop);
         return wd_check_launch(ctx, "mixed cg iteration");
+    };
+
+    // even-odd: x_o = b_o / a + D_oe x_e / (2a) and |M^dag b - M^dag M x| for the
+    // systems that met the Schur target; the ones that miss tol resume
+    auto full_check = [&]() -> int {
+        int e;
+        F64Batch chk_rec = {xe, check, inva_d, inv2a_d};
+        F64Batch chk = {xe, check, ddiag, nullptr};
+        hipLaunchKernelGGL(k_mx_markconv, dim3((n + 255) / 256), dim3(256), 0, ctx->stream,
+                           (const MxState *)st, n, check);
+        if ((e = wd_check_launch(ctx, "k_mx_markconv")) ||
+            (e = wd_launch_hop_f64_batch_c(ctx, xd, xd, Ud, 0, 1, HOP_M, 0.0, bd, nullptr,
+                                           nullptr, nullptr, chk_rec, n)) ||
+            (e = wd_launch_hop_f64_batch(ctx, t64d, xd, Ud, 0, -1, HOP_M, xd, nullptr, nullptr,
+                                         nullptr, chk, n)) ||
+            (e = wd_launch_hop_f64_batch(ctx, q64d, t64d, Ud, 1, -1, HOP_M, t64d, nullptr,
+                                         nullptr, nullptr, chk, n)))
+            return e;
+        hipLaunchKernelGGL(k_mx_fullres, dim3(eg_full, n), dim3(kBlock), 0, ctx->stream,
+                           (const double2 *)Mfull.f->data, (const double2 *)q64d, xe, stride,
+                           (const int *)check, part);
+        hipLaunchKernelGGL(k_mx_fullfin, dim3(n), dim3(1024), 0, ctx->stream,
+                           (const double2 *)part, eg_full, st, active, check, prm->tol,
+                           prm->max_iter);
+        hipLaunchKernelGGL(k_mx_alldone, dim3(1), dim3(64), 0, ctx->stream,
+                           (const int *)active, n, stop);
+        return wd_check_launch(ctx, "mixed cg full check");
     };
 
     const int chunk = prm->check_every;
@@ -644,8 +821,11 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
         for (int k = 0; k < chunk && !e; ++k)
             e = iteration();
         hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
-        if (e)
+        if (e) {
+            if (graph)
+                (void)hipGraphDestroy(graph);
             return e;
+        }
         WD_HIP_CHECK(ctx, ce);
         ce = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
         if (ce != hipSuccess) {
@@ -657,14 +837,26 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
     int status = WD_OK;
     long launched = 0;
     const long limit = (long)prm->max_iter + chunk; // the device forces an exit at max_iter
+    int checks = 0;
+    bool checked = false; // a full check ran since the last iterations
     while (true) {
         int hstop = 0;
         WD_HIP_CHECK(ctx, hipMemcpyAsync(ctx->hpinned, stop, sizeof(int), hipMemcpyDeviceToHost,
                                          ctx->stream));
         WD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
         hstop = *(int *)ctx->hpinned;
+        if (hstop && eo && !checked && checks < 4) {
+            // all systems stopped: verify the converged ones on the full system;
+            // systems that miss tol are resumed and clear the stop flag
+            ++checks;
+            checked = true;
+            if ((status = full_check()))
+                break;
+            continue;
+        }
         if (hstop || launched > limit)
             break;
+        checked = false;
         if (gexec) {
             hipError_t ge = hipGraphLaunch(gexec, ctx->stream);
             if (ge != hipSuccess) {
@@ -693,16 +885,19 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
         *seconds = std::chrono::duration<double>(t1 - t0).count();
     bool all_conv = true, breakdown = false;
     for (int r = 0; r < n; ++r) {
+        const double bn = eo ? hs[r].full_b : hs[r].b_norm;
+        const double tr = eo ? hs[r].full_rsd : hs[r].true_rsd;
+        const bool conv = hs[r].status == 1 && (!eo || !(bn > 0.0) || tr <= prm->tol * bn);
         if (reports) {
             reports[r].iterations = hs[r].iter;
             reports[r].reliable_updates = hs[r].updates;
-            reports[r].converged = hs[r].status == 1;
-            reports[r].true_rel_rsd = hs[r].b_norm > 0 ? hs[r].true_rsd / hs[r].b_norm : 0.0;
-            reports[r].rhs_norm = hs[r].b_norm;
+            reports[r].converged = conv;
+            reports[r].true_rel_rsd = bn > 0 ? tr / bn : 0.0;
+            reports[r].rhs_norm = bn;
             reports[r].rhs_per_thread = R;
             reports[r].site_chunk = C;
         }
-        all_conv = all_conv && hs[r].status == 1;
+        all_conv = all_conv && conv;
         breakdown = breakdown || hs[r].status == 2;
     }
     if (breakdown)

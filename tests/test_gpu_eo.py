@@ -111,3 +111,79 @@ def test_cg_eo_matches_graph_and_eager(w):
     assert out[True][0] == out[False][0]
     assert np.array_equal(out[True][1], out[False][1])
     assert np.array_equal(out[True][2], out[False][2])
+
+
+# ---- mixed precision on the Schur system -------------------------------------------
+@pytest.mark.parametrize("sprec,gprec,recon", [("F16", "F32", 12), ("F32", "F32", 18),
+                                                ("F16", "F16", 12)])
+def test_mixed_eo_reaches_fp64_true_residual(w, oracle, sprec, gprec, recon):
+    """Three systems at three kappas, antiperiodic t: every solution meets the
+    reference's FP64 stop test on the full system, re-checked with the oracle."""
+    L = [8, 8, 8, 8]
+    n = 3
+    A = w.gauge_near_identity(L, 61, 0.3)
+    bs = [w.spinor_gaussian(L, 62, r) for r in range(n)]
+    masses = [kappa_to_mass(k) for k in (0.12, 0.125, 0.13)]
+    print(f"mixed eo {sprec}/{gprec}/{recon}", flush=True)
+    with w.Context(L, bc=[1, 1, 1, -1]) as ctx:
+        U = ctx.gauge(A)
+        Us = ctx.gauge(A, prec=getattr(w, gprec), recon=recon)
+        fb = ctx.field(nrhs=n, data=bs)
+        xe, xf = ctx.field(nrhs=n), ctx.field(nrhs=n)
+        eo = ctx.cg_mixed(xe, fb, U, Us, masses, tol=1e-10, spinor_prec=getattr(w, sprec),
+                          even_odd=True, check_every=4)
+        full = ctx.cg_mixed(xf, fb, U, Us, masses, tol=1e-10, spinor_prec=getattr(w, sprec),
+                            check_every=4)
+        xs = [xe.download(r) for r in range(n)]
+    assert eo["status"] == 0
+    Af = [a.copy() for a in A]
+    w.gauge_flip_boundary(L, 3, Af)
+    for r in range(n):
+        s, sf = eo["systems"][r], full["systems"][r]
+        print(r, s["iterations"], sf["iterations"], s["true_rel_rsd"], flush=True)
+        assert s["converged"] and s["true_rel_rsd"] <= 1e-10
+        assert s["iterations"] < sf["iterations"]
+        assert _normal_residual(oracle, xs[r], bs[r], Af, L, masses[r]) < 1.05e-10
+
+
+def test_mixed_eo_masks_and_zero_source(w, oracle):
+    L = [4, 4, 4, 8]
+    n = 4
+    A = w.gauge_near_identity(L, 71, 0.3)
+    bs = [w.spinor_gaussian(L, 72, r) for r in range(n)]
+    bs[2] = np.zeros_like(bs[2])
+    masses = [2.0, 0.5, 0.3, kappa_to_mass(0.13)]
+    with w.Context(L) as ctx:
+        U = ctx.gauge(A)
+        Us = ctx.gauge(A, prec=w.F32, recon=12)
+        fb = ctx.field(nrhs=n, data=bs)
+        x = ctx.field(nrhs=n)
+        res = ctx.cg_mixed(x, fb, U, Us, masses, tol=1e-10, rhs_per_thread=2, check_every=3,
+                           even_odd=True)
+        xs = [x.download(r) for r in range(n)]
+    its = [s["iterations"] for s in res["systems"]]
+    print(its, flush=True)
+    assert res["status"] == 0
+    assert its[0] < its[1] < its[3] and its[2] == 0
+    assert np.all(xs[2] == 0)
+    for r in (0, 1, 3):
+        assert _normal_residual(oracle, xs[r], bs[r], A, L, masses[r]) < 1.05e-10
+
+
+def test_mixed_eo_graph_and_direct_launch_agree(w):
+    L = [4, 4, 4, 8]
+    A = w.gauge_near_identity(L, 81, 0.3)
+    bs = [w.spinor_gaussian(L, 82, r) for r in range(2)]
+    out = []
+    with w.Context(L) as ctx:
+        U = ctx.gauge(A)
+        Us = ctx.gauge(A, prec=w.F16, recon=12)
+        fb = ctx.field(nrhs=2, data=bs)
+        for g in (True, False):
+            x = ctx.field(nrhs=2)
+            res = ctx.cg_mixed(x, fb, U, Us, [0.1, 0.2], use_graph=g, check_every=3,
+                               rhs_per_thread=1, even_odd=True)
+            out.append(([x.download(r) for r in range(2)], res))
+    for r in range(2):
+        assert np.array_equal(out[0][0][r], out[1][0][r])
+        assert out[0][1]["systems"][r]["iterations"] == out[1][1]["systems"][r]["iterations"]
