@@ -245,9 +245,15 @@ int wd_time_op(wd_ctx *ctx, int op, wd_field *out, const wd_field *in, const wd_
  * a block are read W blocks later instead of one time slice later.  Takes
  * effect when a time slice of one parity (L0/2 * L1 * L2 sites) is a multiple
  * of 256 sites and W is below the blocks per slice; else the natural order.
- * Default: columns of 8 z-planes, W = round(8 * L0/2 * L1 / 256).  Every W
+ * Default: columns of 4 z-planes, W = round(4 * L0/2 * L1 / 256).  Every W
  * gives bit-identical results. */
 #define WD_OPT_BLOCK_TILE 2
+/* Parity order of the both-parity FP64 launches (M, M^dagger): 0 = all blocks
+ * of parity 0, then parity 1; 1 (default) = the two parities of each site
+ * block side by side, sharing each link's forward and backward use through
+ * L2 (links then also load through L2 in the both-parity sloppy operator).
+ * Results are bit-identical. */
+#define WD_OPT_PARITY_ORDER 3
 int wd_set_option(wd_ctx *ctx, int option, int value);
 
 #ifdef __cplusplus

@@ -55,6 +55,9 @@ __device__ __forceinline__ void load_spinor(const double2 *__restrict__ base, in
             psi[s][c] = ld(base, f64_idx(s * 3 + c, (unsigned)j, 12));
 }
 
+// NT: non-temporal (each link read once per launch); temporal when the two
+// parities of a site range run side by side and share links through L2
+template <bool NT>
 __device__ __forceinline__ void load_link(const double2 *__restrict__ base, int stride, int j,
                                           cpx<double> (&U)[3][3])
 {
@@ -62,7 +65,8 @@ __device__ __forceinline__ void load_link(const double2 *__restrict__ base, int 
     for (int a = 0; a < 3; ++a)
 #pragma unroll
         for (int b = 0; b < 3; ++b)
-            U[a][b] = ld_nt(base, f64_idx(a * 3 + b, (unsigned)j, 9));
+            U[a][b] = NT ? ld_nt(base, f64_idx(a * 3 + b, (unsigned)j, 9))
+                         : ld(base, f64_idx(a * 3 + b, (unsigned)j, 9));
 }
 
 // Forces the accumulator to be materialised here so the compiler cannot
@@ -77,7 +81,7 @@ __device__ __forceinline__ void pin_acc(cpx<double> (&acc)[4][3])
 }
 
 // One direction pair (forward and backward) of the hop.
-template <int MU, bool DAG>
+template <int MU, bool DAG, bool NTL>
 __device__ __forceinline__ void hop_mu(cpx<double> (&acc)[4][3], const double2 *__restrict__ in_o,
                                        const double2 *__restrict__ U, const Geom &g, int p, int i,
                                        int jf, int jb, double sf, double sb)
@@ -86,7 +90,7 @@ __device__ __forceinline__ void hop_mu(cpx<double> (&acc)[4][3], const double2 *
     {
         cpx<double> psi[4][3], W[3][3], h[2][3], r[2][3];
         load_spinor(in_o, stride, jf, psi);
-        load_link(U + (long)((MU * 2 + p) * 9) * stride, stride, i, W);
+        load_link<NTL>(U + (long)((MU * 2 + p) * 9) * stride, stride, i, W);
         using D = Dir<MU, true, DAG>;
         project<D>(psi, h);
         su3_mul_half<false>(W, h, r);
@@ -96,7 +100,7 @@ __device__ __forceinline__ void hop_mu(cpx<double> (&acc)[4][3], const double2 *
     {
         cpx<double> psi[4][3], W[3][3], h[2][3], r[2][3];
         load_spinor(in_o, stride, jb, psi);
-        load_link(U + (long)((MU * 2 + (1 - p)) * 9) * stride, stride, jb, W);
+        load_link<NTL>(U + (long)((MU * 2 + (1 - p)) * 9) * stride, stride, jb, W);
         using D = Dir<MU, false, DAG>;
         project<D>(psi, h);
         su3_mul_half<true>(W, h, r);
@@ -105,7 +109,11 @@ __device__ __forceinline__ void hop_mu(cpx<double> (&acc)[4][3], const double2 *
     pin_acc(acc);
 }
 
-template <bool DAG, int MODE>
+// IL (both parities only): the two parities of a site block are consecutive
+// logical workgroups, so a link is read for its forward use by one and for
+// its backward use by the other shortly after (temporal link loads); else the
+// first half of the workgroups computes parity 0 and the second parity 1.
+template <bool DAG, int MODE, bool IL>
 // waves_per_eu(1, 1) lets the scheduler keep more of a direction's loads in
 // flight (~150 VGPRs, still 3 waves per SIMD at run time); measured faster
 // than the default budget on MI355X.
@@ -144,12 +152,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 1))) 
             partials += (size_t)rhs * gridDim.x;
     }
     const int lb = logical_block(remap);
-    // both parities: the first half of the logical workgroups computes parity
-    // 0, the second half parity 1 (alternating the parities over one site
-    // range was measured 10 % slower for FP64 on MI355X)
     const int bpp = gridDim.x / nparity;
-    const int p = (nparity == 2) ? lb / bpp : cb_first;
-    const int blk = block_order(lb - (nparity == 2 ? p * bpp : 0), g);
+    int p, blk;
+    if (IL) {
+        p = lb & 1;
+        blk = block_order(lb >> 1, g);
+    } else {
+        p = (nparity == 2) ? lb / bpp : cb_first;
+        blk = block_order(lb - (nparity == 2 ? p * bpp : 0), g);
+    }
     const int i = blk * kBlock + threadIdx.x;
     const bool active = i < g.Vh;
 
@@ -176,28 +187,28 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 1))) 
 
         {
             const int xf = (x + 1 == Lx) ? 0 : x + 1, xb = (x == 0) ? Lx - 1 : x - 1;
-            hop_mu<0, DAG>(acc, in_o, U, g, p, i, row + (xf >> 1), row + (xb >> 1),
+            hop_mu<0, DAG, !IL>(acc, in_o, U, g, p, i, row + (xf >> 1), row + (xb >> 1),
                            x == Lx - 1 ? g.bc[0] : 1.0, x == 0 ? g.bc[0] : 1.0);
         }
         {
             const int s = g.Lxh;
             const int jf = (y + 1 == g.Ly) ? i - (g.Ly - 1) * s : i + s;
             const int jb = (y == 0) ? i + (g.Ly - 1) * s : i - s;
-            hop_mu<1, DAG>(acc, in_o, U, g, p, i, jf, jb, y == g.Ly - 1 ? g.bc[1] : 1.0,
+            hop_mu<1, DAG, !IL>(acc, in_o, U, g, p, i, jf, jb, y == g.Ly - 1 ? g.bc[1] : 1.0,
                            y == 0 ? g.bc[1] : 1.0);
         }
         {
             const int s = g.Lxh * g.Ly;
             const int jf = (z + 1 == g.Lz) ? i - (g.Lz - 1) * s : i + s;
             const int jb = (z == 0) ? i + (g.Lz - 1) * s : i - s;
-            hop_mu<2, DAG>(acc, in_o, U, g, p, i, jf, jb, z == g.Lz - 1 ? g.bc[2] : 1.0,
+            hop_mu<2, DAG, !IL>(acc, in_o, U, g, p, i, jf, jb, z == g.Lz - 1 ? g.bc[2] : 1.0,
                            z == 0 ? g.bc[2] : 1.0);
         }
         {
             const int s = g.Lxh * g.Ly * g.Lz;
             const int jf = (t + 1 == g.Lt) ? i - (g.Lt - 1) * s : i + s;
             const int jb = (t == 0) ? i + (g.Lt - 1) * s : i - s;
-            hop_mu<3, DAG>(acc, in_o, U, g, p, i, jf, jb, t == g.Lt - 1 ? g.bc[3] : 1.0,
+            hop_mu<3, DAG, !IL>(acc, in_o, U, g, p, i, jf, jb, t == g.Lt - 1 ? g.bc[3] : 1.0,
                            t == 0 ? g.bc[3] : 1.0);
         }
 
@@ -276,23 +287,31 @@ int launch_mode(wd_ctx *ctx, double2 *out, const double2 *in, const double2 *U, 
     if (mode == HOP_M_DOT && partials == ctx->partials && (long)grid * nrhs > ctx->max_partials)
         return wd_set_error(ctx, WD_EARG, "hop: too many workgroups for the partials buffer");
     const dim3 gd(grid, nrhs);
+    const bool il = nparity == 2 && ctx->parity_il;
+#define WD_HOP_F64_LAUNCH(M, I)                                                                    \
+    hipLaunchKernelGGL((k_hop_f64<DAG, M, I>), gd, dim3(kBlock), 0, ctx->stream, out, in, U, g,  \
+                       cb_first, nparity, a, hc, same, dotv, partials, ctx->xcd_remap, stop, b)
     switch (mode) {
     case HOP_ONLY:
-        hipLaunchKernelGGL((k_hop_f64<DAG, HOP_ONLY>), gd, dim3(kBlock), 0, ctx->stream, out,
-                           in, U, g, cb_first, nparity, a, hc, same, dotv, partials,
-                           ctx->xcd_remap, stop, b);
+        if (il)
+            WD_HOP_F64_LAUNCH(HOP_ONLY, true);
+        else
+            WD_HOP_F64_LAUNCH(HOP_ONLY, false);
         break;
     case HOP_M:
-        hipLaunchKernelGGL((k_hop_f64<DAG, HOP_M>), gd, dim3(kBlock), 0, ctx->stream, out,
-                           in, U, g, cb_first, nparity, a, hc, same, dotv, partials,
-                           ctx->xcd_remap, stop, b);
+        if (il)
+            WD_HOP_F64_LAUNCH(HOP_M, true);
+        else
+            WD_HOP_F64_LAUNCH(HOP_M, false);
         break;
     default:
-        hipLaunchKernelGGL((k_hop_f64<DAG, HOP_M_DOT>), gd, dim3(kBlock), 0, ctx->stream,
-                           out, in, U, g, cb_first, nparity, a, hc, same, dotv, partials,
-                           ctx->xcd_remap, stop, b);
+        if (il)
+            WD_HOP_F64_LAUNCH(HOP_M_DOT, true);
+        else
+            WD_HOP_F64_LAUNCH(HOP_M_DOT, false);
         break;
     }
+#undef WD_HOP_F64_LAUNCH
     return wd_check_launch(ctx, "k_hop_f64");
 }
 

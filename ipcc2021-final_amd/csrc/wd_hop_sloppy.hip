@@ -180,9 +180,9 @@ __device__ __forceinline__ void dir_rhs(cpx<float> (&acc)[R][4][3], const Sloppy
     const size_t blk = (size_t)gauge_chunks(GF) * gauge_chunk_bytes(GF) * stride;
     const int lpar = FWD ? p : 1 - p;
     const char *Ub = a.U + (size_t)(MU * 2 + lpar) * blk;
-    // one group: each link is read once, stream it past L2; several groups:
-    // the next groups of this site range re-read it from L2 (uniform branch)
-    if (a.groups > 1)
+    // links read once per launch stream past L2; temporal when the next
+    // groups of this site range, or the other parity, re-read them (uniform)
+    if (a.tlinks)
         load_link32<GF, false>(Ub, stride, (unsigned)(FWD ? i : j), W);
     else
         load_link32<GF, true>(Ub, stride, (unsigned)(FWD ? i : j), W);
@@ -474,9 +474,9 @@ __global__ __launch_bounds__(kBlock) void k_hop_sloppy(SloppyArgs a, Geom g, int
 }
 
 template <int SF, int GF, int R, bool DAG>
-void launch_r(wd_ctx *ctx, const SloppyArgs &a, int cb_first, int nparity, int mode, int grid)
+void launch_r(wd_ctx *ctx, const SloppyArgs &a, const Geom &g, int cb_first, int nparity, int mode,
+              int grid)
 {
-    const Geom &g = ctx->geom;
     switch (mode) {
     case HOP_ONLY:
         hipLaunchKernelGGL((k_hop_sloppy<SF, GF, R, DAG, HOP_ONLY>), dim3(grid), dim3(kBlock), 0,
@@ -494,35 +494,35 @@ void launch_r(wd_ctx *ctx, const SloppyArgs &a, int cb_first, int nparity, int m
 }
 
 template <int SF, int GF, bool DAG>
-void launch_gf(wd_ctx *ctx, const SloppyArgs &a, int R, int cb_first, int nparity, int mode,
-               int grid)
+void launch_gf(wd_ctx *ctx, const SloppyArgs &a, const Geom &g, int R, int cb_first, int nparity,
+               int mode, int grid)
 {
     switch (R) {
     case 1:
-        launch_r<SF, GF, 1, DAG>(ctx, a, cb_first, nparity, mode, grid);
+        launch_r<SF, GF, 1, DAG>(ctx, a, g, cb_first, nparity, mode, grid);
         break;
     case 2:
-        launch_r<SF, GF, 2, DAG>(ctx, a, cb_first, nparity, mode, grid);
+        launch_r<SF, GF, 2, DAG>(ctx, a, g, cb_first, nparity, mode, grid);
         break;
     default:
-        launch_r<SF, GF, 4, DAG>(ctx, a, cb_first, nparity, mode, grid);
+        launch_r<SF, GF, 4, DAG>(ctx, a, g, cb_first, nparity, mode, grid);
         break;
     }
 }
 
 template <int SF, bool DAG>
-void launch_sf(wd_ctx *ctx, const SloppyArgs &a, int gf, int R, int cb_first, int nparity,
-               int mode, int grid)
+void launch_sf(wd_ctx *ctx, const SloppyArgs &a, const Geom &g, int gf, int R, int cb_first,
+               int nparity, int mode, int grid)
 {
     switch (gf) {
     case G_F32_18:
-        launch_gf<SF, G_F32_18, DAG>(ctx, a, R, cb_first, nparity, mode, grid);
+        launch_gf<SF, G_F32_18, DAG>(ctx, a, g, R, cb_first, nparity, mode, grid);
         break;
     case G_F32_12:
-        launch_gf<SF, G_F32_12, DAG>(ctx, a, R, cb_first, nparity, mode, grid);
+        launch_gf<SF, G_F32_12, DAG>(ctx, a, g, R, cb_first, nparity, mode, grid);
         break;
     default:
-        launch_gf<SF, G_F16_12, DAG>(ctx, a, R, cb_first, nparity, mode, grid);
+        launch_gf<SF, G_F16_12, DAG>(ctx, a, g, R, cb_first, nparity, mode, grid);
         break;
     }
 }
@@ -541,7 +541,11 @@ int wd_sloppy_groups(int nrhs, int R) { return (nrhs + R - 1) / R; }
 int wd_launch_hop_sloppy(wd_ctx *ctx, SloppyArgs a, int spinor_prec, int gauge_prec, int recon,
                          int R, int dagger, int cb, int mode)
 {
-    const Geom &g = ctx->geom;
+    // columns of the z-t tiled order twice as wide as the FP64 kernels' (the
+    // FP32/F16 working set per site is smaller; measured faster on MI355X)
+    Geom g = ctx->geom;
+    if (g.tW > 0)
+        g.tW = 2 * g.tW < g.tD ? 2 * g.tW : 0;
     WD_ARG_CHECK(ctx, R == 1 || R == 2 || R == 4, "sloppy hop: rhs per thread must be 1, 2 or 4");
     WD_ARG_CHECK(ctx, a.chunk >= 0, "sloppy hop: site chunk must be >= 0");
     // every per-parity array (at most 6 x 16 bytes per site) must fit 32-bit offsets
@@ -554,20 +558,21 @@ int wd_launch_hop_sloppy(wd_ctx *ctx, SloppyArgs a, int spinor_prec, int gauge_p
     a.bpp = (g.Vh + kBlock - 1) / kBlock;
     a.nsb = a.bpp * nparity;
     a.groups = wd_sloppy_groups(a.nrhs, R);
+    a.tlinks = a.groups > 1 || (nparity == 2 && ctx->parity_il);
     const long grid = (long)a.nsb * a.groups;
     WD_ARG_CHECK(ctx, grid < (1L << 31), "sloppy hop: grid too large");
     const int cb_first = (cb < 0) ? 0 : cb;
     const int gf = gauge_fmt(gauge_prec, recon);
     if (spinor_prec == WD_F16) {
         if (dagger)
-            launch_sf<WD_F16, true>(ctx, a, gf, R, cb_first, nparity, mode, (int)grid);
+            launch_sf<WD_F16, true>(ctx, a, g, gf, R, cb_first, nparity, mode, (int)grid);
         else
-            launch_sf<WD_F16, false>(ctx, a, gf, R, cb_first, nparity, mode, (int)grid);
+            launch_sf<WD_F16, false>(ctx, a, g, gf, R, cb_first, nparity, mode, (int)grid);
     } else {
         if (dagger)
-            launch_sf<WD_F32, true>(ctx, a, gf, R, cb_first, nparity, mode, (int)grid);
+            launch_sf<WD_F32, true>(ctx, a, g, gf, R, cb_first, nparity, mode, (int)grid);
         else
-            launch_sf<WD_F32, false>(ctx, a, gf, R, cb_first, nparity, mode, (int)grid);
+            launch_sf<WD_F32, false>(ctx, a, g, gf, R, cb_first, nparity, mode, (int)grid);
     }
     return wd_check_launch(ctx, "k_hop_sloppy");
 }

@@ -53,6 +53,7 @@ struct wd_ctx {
     void *staging;
     size_t staging_bytes;
     int xcd_remap;        // 1: map workgroups so each XCD owns a contiguous site range
+    int parity_il;        // 1: both-parity FP64 launches alternate the parities per site block
     // autotuned (rhs per thread, chunk) of the sloppy operator, keyed by
     // (nrhs, spinor precision, 100 * gauge precision + recon)
     std::map<std::tuple<int, int, int>, std::pair<int, int>> tuned;

@@ -25,6 +25,7 @@ struct SloppyArgs {
     float one;               // 1.0f, set by the launcher (an operand the compiler cannot fold)
     // filled by the launcher
     int groups, bpp, nsb;
+    int tlinks;              // temporal link loads (links re-read within the launch)
 };
 
 bool wd_sloppy_supported(int spinor_prec, int gauge_prec, int recon);

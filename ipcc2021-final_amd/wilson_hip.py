@@ -30,6 +30,7 @@ F64, F32, F16 = 0, 1, 2
 OP_HOP, OP_M = 0, 1
 OPT_XCD_REMAP = 1
 OPT_BLOCK_TILE = 2
+OPT_PARITY_ORDER = 3
 RECON_18, RECON_12 = 18, 12
 # autotuner candidates (include/wilson.h WD_TUNE_NR x WD_TUNE_NC)
 TUNE_R = (1, 2, 4)

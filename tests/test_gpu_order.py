@@ -18,10 +18,11 @@ L = [16, 16, 8, 4]
 WIDTHS = [0, 1, 3]
 
 
-def _run_fp64(w, width, A, b):
+def _run_fp64(w, width, A, b, parity_order=0):
     out = {}
     with w.Context(L, bc=[1, 1, 1, -1]) as ctx:
         ctx.set_option(w.OPT_BLOCK_TILE, width)
+        ctx.set_option(w.OPT_PARITY_ORDER, parity_order)
         U = ctx.gauge(A)
         fin, fout = ctx.field(data=b), ctx.field()
         for dag in (False, True):
@@ -54,15 +55,28 @@ def test_tiled_order_fp64_bitwise(w, oracle):
         assert rel_l2(runs[3][f"M{dag}"], oracle.dslash(b, Af, L, 0.07, dag)) < 1e-13
 
 
+def test_parity_order_fp64_bitwise(w):
+    """Both parities side by side (shared links through L2) or one after the
+    other: the same values bit for bit, with and without the tiled order."""
+    A = w.gauge_random_su3(L, 87)
+    b = w.spinor_gaussian(L, 88)
+    ref = _run_fp64(w, 0, A, b, 0)
+    for width, order in ((0, 1), (3, 1), (1, 0)):
+        got = _run_fp64(w, width, A, b, order)
+        for k, v in ref.items():
+            assert np.array_equal(np.asarray(got[k]), np.asarray(v)), (width, order, k)
+
+
 def test_tiled_order_sloppy_bitwise(w):
     A = w.gauge_near_identity(L, 79, 0.3)
     n = 5
     bs = [w.spinor_gaussian(L, 80, r) for r in range(n)]
     masses = [0.05 * r for r in range(n)]
     res = {}
-    for W in WIDTHS:
+    for W, order in [(W, 0) for W in WIDTHS] + [(0, 1), (3, 1)]:
         with w.Context(L) as ctx:
             ctx.set_option(w.OPT_BLOCK_TILE, W)
+            ctx.set_option(w.OPT_PARITY_ORDER, order)
             Us = ctx.gauge(A, prec=w.F32, recon=12)
             U = ctx.gauge(A)
             fin = ctx.field(prec=w.F16, nrhs=n, data=bs)
@@ -73,13 +87,14 @@ def test_tiled_order_sloppy_bitwise(w):
             x = ctx.field(nrhs=n)
             mx = ctx.cg_mixed(x, fb, U, Us, masses, tol=1e-10, rhs_per_thread=2, site_chunk=0,
                               check_every=4)
-            res[W] = (o, [x.download(r) for r in range(n)],
-                      [s["iterations"] for s in mx["systems"]])
-    for W in WIDTHS[1:]:
+            res[(W, order)] = (o, [x.download(r) for r in range(n)],
+                               [s["iterations"] for s in mx["systems"]])
+    base = res[(0, 0)]
+    for key, got in res.items():
         for r in range(n):
-            assert np.array_equal(res[W][0][r], res[0][0][r]), (W, r)
-            assert np.array_equal(res[W][1][r], res[0][1][r]), (W, r)
-        assert res[W][2] == res[0][2]
+            assert np.array_equal(got[0][r], base[0][r]), (key, r)
+            assert np.array_equal(got[1][r], base[1][r]), (key, r)
+        assert got[2] == base[2]
 
 
 def test_block_tile_option_errors(w):
