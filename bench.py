@@ -221,6 +221,7 @@ def batched_case(w):
         log("batched: FP64 CG of the 48 systems one by one")
         t_fp = 0.0
         its_fp = []
+        c_fp = np.zeros((len(kappas), L[3]))
         for r in range(n):
             b1 = ctx.field(data=srcs[r])
             x1 = ctx.field()
@@ -228,6 +229,11 @@ def batched_case(w):
                          use_x0=False, check_every=16)
             t_fp += res["seconds"]
             its_fp.append(res["iterations"])
+            c_fp[r // 12] += ctx.pion_correlator(x1, site[3])
+        # the consumer of the 12 point sources per kappa: pion correlator and
+        # effective mass, from the batched mixed Schur solve
+        c_mx = np.array([ctx.pion_correlator(xe, site[3], 12 * k, 12)
+                         for k in range(len(kappas))])
     its = [s["iterations"] for s in mx["systems"]]
     its_e = [s["iterations"] for s in mxe["systems"]]
     V = w.volume(L)
@@ -244,6 +250,14 @@ def batched_case(w):
         "iterations_per_kappa_mixed_even_odd": [int(np.max(its_e[12 * k:12 * k + 12]))
                                                 for k in range(4)],
         "iterations_per_kappa_mixed": [int(np.max(its[12 * k:12 * k + 12])) for k in range(4)],
+        "pion_effective_mass_dt6": [round(float(np.log(c[6] / c[7])), 5) for c in c_mx],
+        # the solves are accurate to 1e-10 relative to the whole solution, so
+        # far from the source (C decays like e^-m dt) only C(0)-relative
+        # differences are meaningful
+        "pion_correlator_max_diff_over_C0_mixed_vs_fp64":
+            float(np.max(np.abs(c_mx - c_fp) / c_fp[:, :1])),
+        "pion_correlator_max_rel_diff_dt_le_8_mixed_vs_fp64":
+            float(np.max(np.abs(c_mx[:, :9] - c_fp[:, :9]) / c_fp[:, :9])),
         "iterations_per_kappa_fp64": [int(np.max(its_fp[12 * k:12 * k + 12])) for k in range(4)],
         "rhs_per_thread": mx["systems"][0]["rhs_per_thread"],
         "site_chunk": mx["systems"][0]["site_chunk"],

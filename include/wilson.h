@@ -228,6 +228,18 @@ int wd_apply_M_sloppy(wd_ctx *ctx, wd_field *out, const wd_field *in, const wd_g
 int wd_tune_report(wd_ctx *ctx, const wd_field *f, const wd_gauge *Us, int *best_R,
                    int *best_chunk, double ms[WD_TUNE_NR * WD_TUNE_NC]);
 
+/* ---- propagators and contractions ----------------------------------------------
+ * The consumer of point-source solves in the reference's Python model
+ * (refs/python/main.py:50-65, refs/python/contraction.py:25-45). */
+
+/* Pion correlator of the point-source propagators S_r, r = rhs_first ..
+ * rhs_first + rhs_count - 1 (the 12 spin x colour solutions of one source,
+ * F64 field): Ct[dt] = sum_r sum_{x : t(x) = (t0 + dt) mod Lt} |S_r(x)|^2 for
+ * dt = 0 .. Lt-1 (host array, Lt entries), t0 the source time slice.
+ * Deterministic. */
+int wd_pion_correlator(wd_ctx *ctx, const wd_field *S, int rhs_first, int rhs_count, int t0,
+                       double *Ct);
+
 /* ---- measurement and tuning --------------------------------------------------
  * No counterpart in the reference, which times whole phases with MPI_Wtime
  * (src/main.cpp:88-110). */

@@ -47,6 +47,7 @@ EXPORTED = [
     "wd_norm2", "wd_dot",
     "wd_cg_params_default", "wd_cg", "wd_cg_eo", "wd_time_op", "wd_set_option",
     "wd_mixed_params_default", "wd_cg_mixed", "wd_apply_M_sloppy", "wd_tune_report",
+    "wd_pion_correlator",
     "wdg_gauge_random_su3", "wdg_gauge_near_identity", "wdg_gauge_unit",
     "wdg_gauge_flip_boundary", "wdg_spinor_gaussian", "wdg_spinor_point", "wdg_philox4x32",
 ]
@@ -154,6 +155,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "wd_apply_M_sloppy": ([vp, vp, vp, vp, dp, i, i, i], i),
         "wd_tune_report": ([vp, vp, vp, ctypes.POINTER(i), ctypes.POINTER(i), dp], i),
         "wd_time_op": ([vp, i, vp, vp, vp, d, i, i, i, ctypes.POINTER(d)], i),
+        "wd_pion_correlator": ([vp, vp, i, i, i, dp], i),
         "wd_set_option": ([vp, i, i], i),
         "wdg_gauge_random_su3": ([ip, ctypes.c_uint64, ctypes.POINTER(dp)], None),
         "wdg_gauge_near_identity": ([ip, ctypes.c_uint64, d, ctypes.POINTER(dp)], None),
@@ -366,6 +368,16 @@ class Context:
                          "site_chunk": r.site_chunk}
                         for r in reps],
         }
+
+    def pion_correlator(self, S, t0: int, rhs_first: int = 0, rhs_count: Optional[int] = None):
+        """C(dt), dt = 0 .. Lt-1, of the point-source propagators S[rhs_first :
+        rhs_first + rhs_count] (the 12 spin x colour solutions of one source)."""
+        if rhs_count is None:
+            rhs_count = S.nrhs - rhs_first
+        Ct = np.zeros(self.L[3])
+        self.check(self.lib.wd_pion_correlator(self.h, S.h, rhs_first, rhs_count, t0, _ptr(Ct)),
+                   "wd_pion_correlator")
+        return Ct
 
     def set_option(self, option: int, value: int):
         self.check(self.lib.wd_set_option(self.h, option, value), "wd_set_option")
