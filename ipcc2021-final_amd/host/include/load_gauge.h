@@ -5,7 +5,9 @@
  *   "RANDOM"  left untouched (the reference's stub; zero links for a fresh
  *             lattice_gauge)
  *   path      raw big-endian FP64 file, 72 doubles per site in lexicographic
- *             order, [mu][row][col][re,im] per site, reordered to even-odd.
+ *             order, [mu][row][col][re,im] per site, reordered to even-odd;
+ *             a file that starts with a NERSC header or a LIME record is read
+ *             as NERSC or ILDG (gauge_io.h), with its checks.
  * Returns 0, or exits with EXIT_FAILURE when the file cannot be opened or is
  * too short (the reference exits on open failure, src/load_gauge.cpp:104-108).
  */

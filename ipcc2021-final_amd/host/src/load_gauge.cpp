@@ -8,6 +8,7 @@
 #include <iostream>
 #include <vector>
 
+#include "gauge_io.h"
 #include "operator.h"
 
 namespace {
@@ -41,6 +42,27 @@ int load_gauge(std::complex<double> *gauge[4], std::string &filename, int *subgs
         std::cerr << "error: can not open file: " << filename << std::endl;
         std::exit(EXIT_FAILURE);
     }
+    // NERSC (text header) and ILDG (LIME container) files are recognised by
+    // their first bytes and read by gauge_io; anything else is the
+    // reference's raw format
+    unsigned char head[12] = {0};
+    const size_t got = std::fread(head, 1, sizeof head, fh);
+    const bool nersc = got == 12 && std::memcmp(head, "BEGIN_HEADER", 12) == 0;
+    const bool lime = got >= 4 && head[0] == 0x45 && head[1] == 0x67 && head[2] == 0x89 &&
+                      head[3] == 0xab;
+    if (nersc || lime) {
+        std::fclose(fh);
+        gauge_file_info info;
+        const int rc = nersc ? read_nersc_gauge(gauge, filename, site_vec, &info)
+                             : read_ildg_gauge(gauge, filename, site_vec, &info);
+        if (rc != GAUGE_IO_OK) {
+            std::cerr << "error: " << (nersc ? "NERSC" : "ILDG") << " gauge file " << filename
+                      << " not read (status " << rc << ")" << std::endl;
+            std::exit(EXIT_FAILURE);
+        }
+        return 0;
+    }
+    std::rewind(fh);
     // 72 big-endian doubles per site, lexicographic over the full lattice
     std::vector<unsigned char> row((size_t)72 * 8 * site_vec[0]);
     int x[4];

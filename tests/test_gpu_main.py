@@ -41,8 +41,25 @@ def parse(text):
     return hist, (int(conv.group(1)) if conv else -1), (float(conv.group(2)) if conv else None), rep
 
 
-@pytest.mark.parametrize("name", MAINS)
-def test_wilson_main_matches_reference_driver(name, tmp_path):
+CASES = [(k, "raw") for k in MAINS] + [(k, c) for k in MAINS
+                                        if MANIFEST["fixtures"][k]["gauge"] == "file"
+                                        for c in ("nersc", "ildg")]
+
+
+def _write_container(kind, path, A, L):
+    """The same links as a NERSC (3x3, IEEE64BIG) or ILDG (64-bit) file: both
+    carry the raw file's payload bytes inside their own header / records."""
+    import ctypes
+    api = ctypes.CDLL(os.path.join(ROOT, "ipcc2021-final_amd", "libipcc_api.so"))
+    Lc = (ctypes.c_int * 4)(*L)
+    ptrs = (ctypes.c_void_p * 4)(*[a.ctypes.data for a in A])
+    rc = api.ipcc_gauge_write(0 if kind == "nersc" else 1, path.encode(), Lc, ptrs, 3, 8, 1)
+    assert rc == 0
+    return path
+
+
+@pytest.mark.parametrize("name,container", CASES)
+def test_wilson_main_matches_reference_driver(name, container, tmp_path):
     meta = MANIFEST["fixtures"][name]
     L = meta["lattice"]
     gauge = "UNIT"
@@ -54,6 +71,10 @@ def test_wilson_main_matches_reference_driver(name, tmp_path):
         A = w.gauge_near_identity(L, meta["gauge_seed"], 0.3)
         gauge = write_gauge_file(str(tmp_path / "gauge.bin"), A, L)
         assert hashlib.sha256(open(gauge, "rb").read()).hexdigest() == meta["gauge_file_sha256"]
+        if container != "raw":
+            raw = open(gauge, "rb").read()
+            gauge = _write_container(container, str(tmp_path / f"gauge.{container}"), A, L)
+            assert raw in open(gauge, "rb").read()  # the same payload bytes
     args = [EXE, repr(meta["mass"]), gauge, *map(str, L), *map(str, L), str(meta["seed"])]
     print(" ".join(args), flush=True)
     p = subprocess.run(args, capture_output=True, text=True, timeout=300)
