@@ -182,6 +182,9 @@ typedef struct wd_mixed_params {
     int even_odd;       /* 1: iterate on the Schur complement of the even sites (as
                            wd_cg_eo; x_o reconstructed, tol checked on the full system
                            with the FP64 operator), 0: on the full M^dag M */
+    int reliable_every; /* iterations between the FP64 reliable-update slots; a system
+                           whose update falls due waits, without iterating, for the
+                           next slot (iteration counts do not change); 0 = automatic */
 } wd_mixed_params;
 
 typedef struct wd_mixed_report {
@@ -196,7 +199,8 @@ typedef struct wd_mixed_report {
 } wd_mixed_report;
 
 /* max_iter 100000, tol 1e-10, delta 1e-2, F16 spinors, autotuned rhs per
- * thread and site chunk, check_every 16, graphs on, full system (even_odd 0). */
+ * thread and site chunk, check_every 16, graphs on, full system (even_odd 0),
+ * automatic reliable-update slots (reliable_every 0). */
 void wd_mixed_params_default(wd_mixed_params *p);
 
 /* Solves M_r^dag M_r x_r = M_r^dag b_r for all nrhs systems of b at once

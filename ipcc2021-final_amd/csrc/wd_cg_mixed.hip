@@ -11,22 +11,29 @@
 // lattice QCD by Clark et al., Comput. Phys. Commun. 181 (2010) 1517).  A
 // system has converged when the FP64 true residual is below tol * |M^dagger b|.
 //
-// Every system carries its own scalars and flags in device memory; one
-// iteration is a fixed sequence of launches in which converged systems and
-// systems without a pending reliable update are skipped by whole workgroups,
-// so the host only polls the flags every `check_every` iterations and the
-// sequence can be replayed from a captured hipGraph:
+// Every system carries its own scalars and flags in device memory; the
+// launches form a fixed sequence in which converged systems and systems
+// without a pending reliable update are skipped by whole workgroups, so the
+// host only polls the flags every `check_every` iterations and the sequence
+// can be replayed from a captured hipGraph.  An iteration:
 //   xpay    p = r_s + beta p
-//   M       qq = M p                      (sloppy, all active systems in one launch)
+//   M       qq = M p                      (sloppy, all iterating systems in one launch)
 //   M^dag   q = M^dag qq, partial (p, q)  (sloppy)
 //   alpha   alpha = rho / (p, q)
 //   update  x_s += alpha p, r_s -= alpha q, partial |r_s|^2
-//   decide  reliable update due?  else beta = |r_s|^2 / rho, rho = |r_s|^2
-//   relx    x += x_s, x_s = 0                             (pending systems)
-//   M, M^dag (FP64) on x                                   (pending systems)
-//   relr    r = M^dag b - M^dag M x -> r_s, partial |r|^2 (pending systems)
-//   relfin  converged?  else beta = |r|^2 / rho, rho = |r|^2
-//   alldone stop flag when no system is active
+//   decide  reliable update due?  then the system waits (active = 2), else
+//           beta = |r_s|^2 / rho, rho = |r_s|^2
+// and every `reliable_every` iterations (and at the end of each block) a
+// reliable-update slot:
+//   relx    x += x_s, x_s = 0                             (waiting systems)
+//   M, M^dag (FP64) on x                                   (waiting systems)
+//   relr    r = M^dag b - M^dag M x -> r_s, partial |r|^2 (waiting systems)
+//   relfin  converged?  else beta = |r|^2 / rho, rho = |r|^2, iterate again
+//   alldone stop flag when no system is iterating or waiting
+// A waiting system skips the iterations up to the slot, so its iterates and
+// iteration count are those of an update in the same iteration; the slots
+// only save the launches (each a full grid of skipped workgroups) that an
+// update check after every iteration would cost.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -68,15 +75,15 @@ template <int SF>
 __global__ __launch_bounds__(kBlock) void k_mx_xpay(char *__restrict__ p, float *__restrict__ ps,
                                                     const char *__restrict__ r,
                                                     const float *__restrict__ rs,
-                                                    size_t rhs_bytes, int stride,
+                                                    size_t rhs_bytes, int stride, int nel,
                                                     const MxState *__restrict__ st,
                                                     const int *__restrict__ active)
 {
     const int rhs = blockIdx.y;
-    if (!active[rhs])
+    if (active[rhs] != 1)
         return;
     const int e = blockIdx.x * kBlock + threadIdx.x;
-    if (e >= 2 * stride)
+    if (e >= nel)
         return;
     const int par = e / stride, i = e - par * stride;
     const size_t off = rhs * rhs_bytes + (size_t)par * spinor_chunks(SF) * 16 * stride;
@@ -98,15 +105,15 @@ template <int SF>
 __global__ __launch_bounds__(kBlock) void k_mx_update(
     char *__restrict__ xs, char *__restrict__ r, float *__restrict__ rs,
     const char *__restrict__ p, const float *__restrict__ ps, const char *__restrict__ q,
-    const float *__restrict__ qs, size_t rhs_bytes, size_t xs_rhs_bytes, int stride,
+    const float *__restrict__ qs, size_t rhs_bytes, size_t xs_rhs_bytes, int stride, int nel,
     const MxState *__restrict__ st, const int *__restrict__ active, double2 *__restrict__ partials)
 {
     const int rhs = blockIdx.y;
-    if (!active[rhs])
+    if (active[rhs] != 1)
         return;
     const int e = blockIdx.x * kBlock + threadIdx.x;
     double acc = 0.0;
-    if (e < 2 * stride) {
+    if (e < nel) {
         const int par = e / stride, i = e - par * stride;
         const size_t off = rhs * rhs_bytes + (size_t)par * spinor_chunks(SF) * 16 * stride;
         const size_t xoff = rhs * xs_rhs_bytes + (size_t)par * spinor_chunks(WD_F32) * 16 * stride;
@@ -153,14 +160,14 @@ __global__ __launch_bounds__(kBlock) void k_mx_update(
 // x (F64) += x_s (F32); x_s = 0
 __global__ __launch_bounds__(kBlock) void k_mx_relx(double2 *__restrict__ x,
                                                     char *__restrict__ xs, size_t x_rhs_elems,
-                                                    size_t xs_rhs_bytes, int stride,
+                                                    size_t xs_rhs_bytes, int stride, int nel,
                                                     const int *__restrict__ reliable)
 {
     const int rhs = blockIdx.y;
     if (!reliable[rhs])
         return;
     const int e = blockIdx.x * kBlock + threadIdx.x;
-    if (e >= 2 * stride)
+    if (e >= nel)
         return;
     const int par = e / stride, i = e - par * stride;
     double2 *xp = x + rhs * x_rhs_elems + (size_t)par * 12 * stride;
@@ -188,7 +195,8 @@ __global__ __launch_bounds__(kBlock) void k_mx_relr(char *__restrict__ rsl, floa
                                                     const double2 *__restrict__ bp,
                                                     const double2 *__restrict__ q64,
                                                     size_t x_rhs_elems, size_t rhs_bytes,
-                                                    int stride, const int *__restrict__ reliable,
+                                                    int stride, int nel,
+                                                    const int *__restrict__ reliable,
                                                     double2 *__restrict__ partials)
 {
     const int rhs = blockIdx.y;
@@ -196,7 +204,7 @@ __global__ __launch_bounds__(kBlock) void k_mx_relr(char *__restrict__ rsl, floa
         return;
     const int e = blockIdx.x * kBlock + threadIdx.x;
     double acc = 0.0;
-    if (e < 2 * stride) {
+    if (e < nel) {
         const int par = e / stride, i = e - par * stride;
         const size_t xo = rhs * x_rhs_elems + (size_t)par * 12 * stride;
         cpx<double> vb[4][3], vq[4][3];
@@ -256,7 +264,7 @@ __global__ __launch_bounds__(1024) void k_mx_alpha(const double2 *__restrict__ p
                                                    int *__restrict__ active)
 {
     const int r = blockIdx.x;
-    if (!active[r])
+    if (active[r] != 1)
         return;
     const double2 pq = sum_rhs_partials(partials, count);
     if (threadIdx.x == 0) {
@@ -271,12 +279,12 @@ __global__ __launch_bounds__(1024) void k_mx_alpha(const double2 *__restrict__ p
 
 __global__ __launch_bounds__(1024) void k_mx_decide(const double2 *__restrict__ partials,
                                                     int count, MxState *__restrict__ st,
-                                                    const int *__restrict__ active,
+                                                    int *__restrict__ active,
                                                     int *__restrict__ reliable, int max_iter,
                                                     double delta)
 {
     const int r = blockIdx.x;
-    if (!active[r])
+    if (active[r] != 1)
         return;
     const double2 s = sum_rhs_partials(partials, count);
     if (threadIdx.x == 0) {
@@ -286,6 +294,7 @@ __global__ __launch_bounds__(1024) void k_mx_decide(const double2 *__restrict__ 
         m.rsd = rsd;
         if (rsd < delta * m.maxr || rsd < m.target || m.iter >= max_iter || !isfinite(rr)) {
             reliable[r] = 1;
+            active[r] = 2; // waits for the next reliable-update slot
         } else {
             m.beta = rr / m.rho;
             m.rho = rr;
@@ -318,6 +327,7 @@ __global__ __launch_bounds__(1024) void k_mx_relfin(const double2 *__restrict__ 
             m.beta = rr / m.rho;
             m.rho = rr;
             m.maxr = rsd;
+            active[r] = 1;
         }
     }
 }
@@ -522,6 +532,7 @@ extern "C" void wd_mixed_params_default(wd_mixed_params *p)
     p->check_every = 16;
     p->use_graph = 1;
     p->even_odd = 0;
+    p->reliable_every = 0;
 }
 
 extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd_gauge *U,
@@ -545,7 +556,7 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
                  "mixed cg: rhs_per_thread must be 0, 1, 2 or 4");
     WD_ARG_CHECK(ctx, prm->site_chunk >= -1, "mixed cg: site_chunk must be >= -1");
     WD_ARG_CHECK(ctx, prm->max_iter >= 1 && prm->check_every >= 1 && prm->tol > 0.0 &&
-                          prm->delta > 0.0 && prm->delta < 1.0,
+                          prm->delta > 0.0 && prm->delta < 1.0 && prm->reliable_every >= 0,
                  "mixed cg: bad iteration / tolerance parameters");
     const auto t0 = std::chrono::steady_clock::now();
     const int n = b->nrhs;
@@ -564,7 +575,10 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
         (eo && (rc = wd_field_create(ctx, &Mfull.f, WD_F64, n))))
         return rc;
     const int eg_full = (2 * stride + kBlock - 1) / kBlock; // elementwise workgroups per rhs
-    const int eg = eo ? (stride + kBlock - 1) / kBlock : eg_full; // (even half only)
+    // elementwise kernels cover (parity, site) e < nel: the even half only for
+    // even-odd (the odd half of the work vectors holds intermediate hops)
+    const int nel = eo ? stride : 2 * stride;
+    const int eg = (nel + kBlock - 1) / kBlock;
     const int bpp = (int)((ctx->Vh + kBlock - 1) / kBlock);
     const int hop_nsb = eo ? bpp : 2 * bpp; // partials of the (last) operator launch
     const int nsb_max = std::max(eg_full, 2 * bpp);
@@ -645,11 +659,11 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
     if (sp == WD_F16)
         hipLaunchKernelGGL(k_mx_relr<WD_F16>, dim3(eg, n), dim3(kBlock), 0, ctx->stream,
                            (char *)rs.f->data, rs.f->scale, (const double2 *)Mdbd,
-                           (const double2 *)nullptr, xe, srb, stride, (const int *)nullptr, part);
+                           (const double2 *)nullptr, xe, srb, stride, nel, (const int *)nullptr, part);
     else
         hipLaunchKernelGGL(k_mx_relr<WD_F32>, dim3(eg, n), dim3(kBlock), 0, ctx->stream,
                            (char *)rs.f->data, rs.f->scale, (const double2 *)Mdbd,
-                           (const double2 *)nullptr, xe, srb, stride, (const int *)nullptr, part);
+                           (const double2 *)nullptr, xe, srb, stride, nel, (const int *)nullptr, part);
     if ((rc = wd_check_launch(ctx, "k_mx_relr")))
         return rc;
     // even-odd: the Schur target starts at half the full-system tolerance
@@ -724,12 +738,12 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
         if (sp == WD_F16)
             hipLaunchKernelGGL(k_mx_xpay<WD_F16>, dim3(eg, n), dim3(kBlock), 0, ctx->stream,
                                (char *)p.f->data, p.f->scale, (const char *)rs.f->data,
-                               (const float *)rs.f->scale, srb, stride, (const MxState *)st,
+                               (const float *)rs.f->scale, srb, stride, nel, (const MxState *)st,
                                (const int *)active);
         else
             hipLaunchKernelGGL(k_mx_xpay<WD_F32>, dim3(eg, n), dim3(kBlock), 0, ctx->stream,
                                (char *)p.f->data, p.f->scale, (const char *)rs.f->data,
-                               (const float *)rs.f->scale, srb, stride, (const MxState *)st,
+                               (const float *)rs.f->scale, srb, stride, nel, (const MxState *)st,
                                (const int *)active);
         if ((e = wd_check_launch(ctx, "k_mx_xpay")))
             return e;
@@ -743,18 +757,24 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
                                (char *)xs.f->data, (char *)rs.f->data, rs.f->scale,
                                (const char *)p.f->data, (const float *)p.f->scale,
                                (const char *)q.f->data, (const float *)q.f->scale, srb, xsb,
-                               stride, (const MxState *)st, (const int *)active, part);
+                               stride, nel, (const MxState *)st, (const int *)active, part);
         else
             hipLaunchKernelGGL(k_mx_update<WD_F32>, dim3(eg, n), dim3(kBlock), 0, ctx->stream,
                                (char *)xs.f->data, (char *)rs.f->data, rs.f->scale,
                                (const char *)p.f->data, (const float *)p.f->scale,
                                (const char *)q.f->data, (const float *)q.f->scale, srb, xsb,
-                               stride, (const MxState *)st, (const int *)active, part);
+                               stride, nel, (const MxState *)st, (const int *)active, part);
         hipLaunchKernelGGL(k_mx_decide, dim3(n), dim3(1024), 0, ctx->stream,
-                           (const double2 *)part, eg, st, (const int *)active, reliable,
-                           prm->max_iter, prm->delta);
+                           (const double2 *)part, eg, st, active, reliable, prm->max_iter,
+                           prm->delta);
+        return wd_check_launch(ctx, "mixed cg iteration");
+    };
+
+    // reliable-update slot for the waiting systems
+    auto reliable_slot = [&]() -> int {
+        int e;
         hipLaunchKernelGGL(k_mx_relx, dim3(eg, n), dim3(kBlock), 0, ctx->stream, xd,
-                           (char *)xs.f->data, xe, xsb, stride, (const int *)reliable);
+                           (char *)xs.f->data, xe, xsb, stride, nel, (const int *)reliable);
         if ((e = wd_check_launch(ctx, "k_mx_relx")))
             return e;
         if (!eo) {
@@ -771,18 +791,34 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
         if (sp == WD_F16)
             hipLaunchKernelGGL(k_mx_relr<WD_F16>, dim3(eg, n), dim3(kBlock), 0, ctx->stream,
                                (char *)rs.f->data, rs.f->scale, (const double2 *)Mdbd,
-                               (const double2 *)q64d, xe, srb, stride, (const int *)reliable,
+                               (const double2 *)q64d, xe, srb, stride, nel, (const int *)reliable,
                                part);
         else
             hipLaunchKernelGGL(k_mx_relr<WD_F32>, dim3(eg, n), dim3(kBlock), 0, ctx->stream,
                                (char *)rs.f->data, rs.f->scale, (const double2 *)Mdbd,
-                               (const double2 *)q64d, xe, srb, stride, (const int *)reliable,
+                               (const double2 *)q64d, xe, srb, stride, nel, (const int *)reliable,
                                part);
         hipLaunchKernelGGL(k_mx_relfin, dim3(n), dim3(1024), 0, ctx->stream,
                            (const double2 *)part, eg, st, active, reliable, prm->max_iter);
         hipLaunchKernelGGL(k_mx_alldone, dim3(1), dim3(64), 0, ctx->stream,
                            (const int *)active, n, stop);
-        return wd_check_launch(ctx, "mixed cg iteration");
+        return wd_check_launch(ctx, "mixed cg reliable update");
+    };
+
+    // a block of `chunk` iterations with a reliable-update slot after every
+    // `slot_every` of them and after the last
+    const int chunk = prm->check_every;
+    // automatic spacing (measured on MI355X): 8 for one system, 4 for many,
+    // where updates of different systems share the FP64 launches of a slot
+    const int slot_every = prm->reliable_every > 0 ? prm->reliable_every : (n == 1 ? 8 : 4);
+    auto block = [&]() -> int {
+        int e = 0;
+        for (int k = 0; k < chunk && !e; ++k) {
+            e = iteration();
+            if (!e && ((k + 1) % slot_every == 0 || k + 1 == chunk))
+                e = reliable_slot();
+        }
+        return e;
     };
 
     // even-odd: x_o = b_o / a + D_oe x_e / (2a) and |M^dag b - M^dag M x| for the
@@ -812,14 +848,11 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
         return wd_check_launch(ctx, "mixed cg full check");
     };
 
-    const int chunk = prm->check_every;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     if (prm->use_graph) {
         WD_HIP_CHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        int e = 0;
-        for (int k = 0; k < chunk && !e; ++k)
-            e = iteration();
+        const int e = block();
         hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
         if (e) {
             if (graph)
@@ -865,9 +898,7 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
                 break;
             }
         } else {
-            for (int k = 0; k < chunk && !status; ++k)
-                status = iteration();
-            if (status)
+            if ((status = block()))
                 break;
         }
         launched += chunk;

@@ -9,8 +9,8 @@
 // traffic is shared R ways in registers; the G = ceil(nrhs/R) groups of one
 // site range are consecutive logical workgroups and land on the same XCD, so
 // the remaining link re-reads hit that XCD's L2.  Right-hand sides whose
-// `active` flag is 0 (converged systems) are skipped per workgroup (uniform
-// branch).  Per-rhs diagonal coefficients a_r = 4 + m_r allow several masses
+// `active` flag is not 1 (converged systems, systems waiting for a reliable
+// update) are skipped per workgroup (uniform branch).  Per-rhs diagonal coefficients a_r = 4 + m_r allow several masses
 // in one launch.
 //
 // Instruction budget: with F16 spinors the kernel issues about as many VALU
@@ -318,7 +318,7 @@ __global__ __launch_bounds__(kBlock) void k_hop_sloppy(SloppyArgs a, Geom g, int
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         rhs[r] = grp * R + r;
-        act[r] = rhs[r] < a.nrhs && (a.active == nullptr || a.active[rhs[r]] != 0);
+        act[r] = rhs[r] < a.nrhs && (a.active == nullptr || a.active[rhs[r]] == 1);
         any = any || act[r];
     }
     if (!any)

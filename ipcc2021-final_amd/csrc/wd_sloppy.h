@@ -16,7 +16,7 @@ struct SloppyArgs {
     size_t rhs_bytes;        // spinor bytes per rhs (all fields share the format)
     const char *U;           // gauge field
     const float *diag;       // [nrhs] a_r = 4 + m_r
-    const int *active;       // [nrhs] 0 = skip this rhs, nullptr = all active
+    const int *active;       // [nrhs] 1 = apply to this rhs, else skip; nullptr = all
     double2 *partials;       // [nrhs][nsb] for HOP_M_DOT
     const int *stop;         // whole launch becomes a no-op when *stop != 0
     int nrhs;

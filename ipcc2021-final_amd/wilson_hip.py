@@ -94,6 +94,7 @@ class MixedParams(ctypes.Structure):
         ("check_every", ctypes.c_int),
         ("use_graph", ctypes.c_int),
         ("even_odd", ctypes.c_int),
+        ("reliable_every", ctypes.c_int),
     ]
 
 
@@ -346,12 +347,13 @@ class Context:
     def cg_mixed(self, x, b, U, Us, masses, tol: float = 1e-10, delta: float = 1e-2,
                  spinor_prec: int = F16, rhs_per_thread: int = 0, site_chunk: int = -1,
                  max_iter: int = 100000, check_every: int = 16, use_graph: bool = True,
-                 even_odd: bool = False):
+                 even_odd: bool = False, reliable_every: int = 0):
         p = MixedParams()
         self.lib.wd_mixed_params_default(ctypes.byref(p))
         p.max_iter, p.tol, p.delta = max_iter, tol, delta
         p.spinor_prec, p.rhs_per_thread, p.site_chunk = spinor_prec, rhs_per_thread, site_chunk
         p.check_every, p.use_graph, p.even_odd = check_every, int(use_graph), int(even_odd)
+        p.reliable_every = reliable_every
         m = np.ascontiguousarray(masses, dtype=np.float64)
         assert m.size == b.nrhs
         reps = (MixedReport * b.nrhs)()

@@ -146,6 +146,35 @@ def test_mixed_eo_reaches_fp64_true_residual(w, oracle, sprec, gprec, recon):
         assert _normal_residual(oracle, xs[r], bs[r], Af, L, masses[r]) < 1.05e-10
 
 
+@pytest.mark.parametrize("L", [[4, 4, 6, 8], [6, 6, 6, 6]])
+@pytest.mark.parametrize("sprec", ["F16", "F32"])
+def test_mixed_eo_half_volume_not_a_multiple_of_the_block(w, oracle, L, sprec):
+    """Half volumes of 384 and 648 sites: the last workgroup of a parity runs
+    past the even half (regression: the elementwise kernels once spilled into
+    the odd half there, corrupting the Schur residual norms)."""
+    n = 2
+    A = w.gauge_near_identity(L, 81, 0.3)
+    bs = [w.spinor_gaussian(L, 82, r) for r in range(n)]
+    masses = [kappa_to_mass(k) for k in (0.11, 0.125)]
+    print(f"mixed eo L={L} {sprec}", flush=True)
+    with w.Context(L, bc=[1, 1, 1, -1]) as ctx:
+        U = ctx.gauge(A)
+        Us = ctx.gauge(A, prec=w.F16 if sprec == "F16" else w.F32, recon=12)
+        fb = ctx.field(nrhs=n, data=bs)
+        x = ctx.field(nrhs=n)
+        res = ctx.cg_mixed(x, fb, U, Us, masses, tol=1e-10, spinor_prec=getattr(w, sprec),
+                           even_odd=True, check_every=4)
+        xs = [x.download(r) for r in range(n)]
+    assert res["status"] == 0
+    Af = [a.copy() for a in A]
+    w.gauge_flip_boundary(L, 3, Af)
+    for r in range(n):
+        s = res["systems"][r]
+        print(r, s["iterations"], s["true_rel_rsd"], flush=True)
+        assert s["converged"] and s["iterations"] < 120
+        assert _normal_residual(oracle, xs[r], bs[r], Af, L, masses[r]) < 1.05e-10
+
+
 def test_mixed_eo_masks_and_zero_source(w, oracle):
     L = [4, 4, 4, 8]
     n = 4

@@ -166,6 +166,41 @@ def test_mixed_graph_and_direct_launch_agree(w):
         assert out[0][1]["systems"][r]["iterations"] == out[1][1]["systems"][r]["iterations"]
 
 
+@pytest.mark.parametrize("even_odd", [False, True])
+def test_reliable_update_slots_do_not_change_iterates(w, even_odd):
+    """A system whose reliable update falls due waits for the next slot without
+    iterating, so the slot spacing changes neither iterates nor counts: four
+    systems of different masses (updates in different iterations, two per
+    thread so a waiting system shares its workgroups with an iterating one),
+    slots after every iteration, every 3rd, and only at the end of each block
+    of 4 (spacing 5 > block)."""
+    L = [4, 4, 6, 8]
+    A, bs = _setup(w, L, 4)
+    masses = [kappa_to_mass(k) for k in (0.11, 0.12, 0.13, 0.135)]
+    out = []
+    with w.Context(L, bc=[1, 1, 1, -1]) as ctx:
+        U = ctx.gauge(A)
+        Us = ctx.gauge(A, prec=w.F16, recon=12)
+        fb = ctx.field(nrhs=4, data=bs)
+        for K in (1, 3, 5):
+            x = ctx.field(nrhs=4)
+            res = ctx.cg_mixed(x, fb, U, Us, masses, tol=1e-10, rhs_per_thread=2,
+                               check_every=4, even_odd=even_odd, reliable_every=K)
+            out.append(([x.download(r) for r in range(4)], res))
+            x.close()
+    its = [[s["iterations"] for s in o[1]["systems"]] for o in out]
+    ups = [[s["reliable_updates"] for s in o[1]["systems"]] for o in out]
+    print("iterations", its, "updates", ups, flush=True)
+    assert len(set(its[0])) > 1  # the systems do stop at different iterations
+    for k in (1, 2):
+        assert its[k] == its[0] and ups[k] == ups[0]
+        assert [s["true_rel_rsd"] for s in out[k][1]["systems"]] == \
+            [s["true_rel_rsd"] for s in out[0][1]["systems"]]
+        for r in range(4):
+            assert np.array_equal(out[k][0][r], out[0][0][r])
+    assert all(s["converged"] for s in out[0][1]["systems"])
+
+
 def test_f16_storage_of_underflowing_sites(w):
     """Sites whose components are far below FP32 normal range (as far from a
     point source) are stored as exact zeros in F16 storage, never as NaN."""
@@ -222,3 +257,6 @@ def test_sloppy_argument_errors(w):
             ctx.apply_M_sloppy(o16, f16, Us, [0.1, 0.1], rhs_per_thread=3)
         with pytest.raises(w.WilsonError):
             ctx.apply_M_sloppy(o16, f16, Us, [0.1, 0.1], site_chunk=-1)
+        with pytest.raises(w.WilsonError):
+            ctx.cg_mixed(out, f64, ctx.gauge(w.gauge_unit(L)), Us, [0.1, 0.1],
+                         reliable_every=-1)
