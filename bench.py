@@ -46,7 +46,8 @@ FLOP_M = 1320 + 48         # per site, hop + (4+m) diagonal (fused FMA)
 BYTES_HOP_F64 = 8 * 144 + 2 * 192
 BYTES_M_F64 = 8 * 144 + 3 * 192
 BYTES_M_SLOPPY = 8 * 48 + 3 * 52   # F32 12-real links, F16 spinors + float scale
-PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_hop_f64_M_32x32x32x64_pmc.txt")
+PMC_SUMMARY = {0: os.path.join(ROOT, "profiles", "k_hop_f64_M_32x32x32x64_pmc.txt"),
+               16: os.path.join(ROOT, "profiles", "k_hop_f64_ts_M_32x32x32x64_pmc.txt")}
 
 
 def load_pkg():
@@ -117,13 +118,15 @@ def cpu_baseline(L, seconds):
     return None
 
 
-def pmc_traffic():
-    """HBM bytes per launch of the step's kernel from the committed PMC summary."""
-    if not os.path.exists(PMC_SUMMARY):
+def pmc_traffic(L, tstream):
+    """HBM bytes per launch of the step's kernel from the committed PMC summary
+    (taken at 32^3x64 for the kernel the context runs)."""
+    path = PMC_SUMMARY.get(tstream)
+    if list(L) != [32, 32, 32, 64] or path is None or not os.path.exists(path):
         return None, None
-    text = open(PMC_SUMMARY).read()
+    text = open(path).read()
     m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", text, re.M)
-    return (float(m.group(1)) if m else None), os.path.relpath(PMC_SUMMARY, ROOT)
+    return (float(m.group(1)) if m else None), os.path.relpath(path, ROOT)
 
 
 # ---- GPU workloads -------------------------------------------------------------------
@@ -379,6 +382,7 @@ def main():
         # the step's kernel and the one-parity hop, HIP events on the library stream
         iters = max(args.steps, 10)
         ms_m = ctx.time_op(w.OP_M, fout, fin, U, mass, False, 0, iters)
+        tstream = ctx.get_option(w.OPT_TSTREAM)
         ms_hop = ctx.time_op(w.OP_HOP, fout, fin, U, mass, False, 0, iters)
         # sloppy operator (the mixed solver's inner kernel), one system
         Us = ctx.gauge(A, prec=w.F32, recon=12)
@@ -399,7 +403,9 @@ def main():
     gflops = FLOP_M * V / (ms_step * 1e-3) / 1e9
     gbs_m = BYTES_M_F64 * V / (ms_m * 1e-3) / 1e9
     gbs_hop = BYTES_HOP_F64 * (V // 2) / (ms_hop * 1e-3) / 1e9
-    traffic, traffic_src = pmc_traffic()
+    traffic, traffic_src = pmc_traffic(L, tstream)
+    m_kernel = (f"k_hop_f64_ts<dagger=0, HOP_M>, t runs of {tstream}" if tstream
+                else "k_hop_f64<dagger=0, HOP_M>")
     line = {
         "metric": "Dslash GB/s & GFLOP/s vs HBM roofline; CG solves/sec at 1e-10, mixed vs FP64",
         "value": round(gflops, 1),
@@ -418,7 +424,7 @@ def main():
                    "step": "one full-lattice M application (fused diagonal + hop, both parities)"},
         "roofline": {"bound": "hbm", "achieved": round(gbs_m, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(gbs_m / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "kernel": "k_hop_f64<dagger=0, HOP_M>",
+                     "traffic": traffic, "kernel": m_kernel,
                      "algorithmic_bytes_per_site": BYTES_M_F64, "sites_per_launch": V,
                      "ms_per_launch": round(ms_m, 4)},
         "hop_roofline": {"kernel": "k_hop_f64<dagger=0, HOP_ONLY>, one parity",

@@ -270,7 +270,21 @@ int wd_time_op(wd_ctx *ctx, int op, wd_field *out, const wd_field *in, const wd_
  * L2 (links then also load through L2 in the both-parity sloppy operator).
  * Results are bit-identical. */
 #define WD_OPT_PARITY_ORDER 3
+/* t-streaming of the both-parity FP64 launches (M, M^dagger and the CG's
+ * M^dagger with its dot product, one system): each thread owns a column
+ * (x, y, z) for a run of T consecutive t, keeping the t link and the t
+ * neighbour of a site in registers for the next site.  0 = off (site-block
+ * kernel); 1 (default) = automatic: T = 16 when Lt is a multiple of 16 and the
+ * launch still has at least 512 workgroups, else off; T >= 2 forces runs of T
+ * (Lt must be a multiple of T).  M and the hop are bit-identical to the
+ * site-block kernel; the dot product's partial sums are grouped differently. */
+#define WD_OPT_TSTREAM 4
 int wd_set_option(wd_ctx *ctx, int option, int value);
+/* The value an option has in effect on this context: 0/1 for
+ * WD_OPT_XCD_REMAP and WD_OPT_PARITY_ORDER, the column width W (0 = natural
+ * order) for WD_OPT_BLOCK_TILE, the t-run length T (0 = site-block kernel)
+ * for WD_OPT_TSTREAM. */
+int wd_get_option(wd_ctx *ctx, int option, int *value);
 
 #ifdef __cplusplus
 }

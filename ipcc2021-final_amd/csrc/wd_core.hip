@@ -124,6 +124,7 @@ extern "C" int wd_create(wd_ctx **out, const int L[4], const int bc[4], int devi
     }
     c->xcd_remap = 1;
     c->parity_il = 1;
+    c->ts_tb = wd_auto_tstream(c);
     // partials: enough for a both-parity hop launch and the streaming
     // reduction grids (kStreamPartials, 256 KiB)
     const int hop_blocks = (int)(2 * ((Vh + kBlock - 1) / kBlock));
@@ -509,6 +510,29 @@ extern "C" int wd_time_op(wd_ctx *ctx, int op, wd_field *out, const wd_field *in
     return WD_OK;
 }
 
+extern "C" int wd_get_option(wd_ctx *ctx, int option, int *value)
+{
+    if (!ctx)
+        return WD_EARG;
+    WD_ARG_CHECK(ctx, value != nullptr, "get_option: null value");
+    switch (option) {
+    case WD_OPT_XCD_REMAP:
+        *value = ctx->xcd_remap;
+        return WD_OK;
+    case WD_OPT_PARITY_ORDER:
+        *value = ctx->parity_il;
+        return WD_OK;
+    case WD_OPT_BLOCK_TILE:
+        *value = ctx->geom.tW;
+        return WD_OK;
+    case WD_OPT_TSTREAM:
+        *value = ctx->ts_tb;
+        return WD_OK;
+    default:
+        return wd_set_error(ctx, WD_EARG, "get_option: unknown option");
+    }
+}
+
 extern "C" int wd_set_option(wd_ctx *ctx, int option, int value)
 {
     if (!ctx)
@@ -519,6 +543,12 @@ extern "C" int wd_set_option(wd_ctx *ctx, int option, int value)
         return WD_OK;
     case WD_OPT_PARITY_ORDER:
         ctx->parity_il = value ? 1 : 0;
+        return WD_OK;
+    case WD_OPT_TSTREAM:
+        WD_ARG_CHECK(ctx, value >= 0, "set_option: t-run length must be >= 0");
+        if (value >= 2)
+            WD_ARG_CHECK(ctx, ctx->L[3] % value == 0, "set_option: Lt must be a multiple of the t-run length");
+        ctx->ts_tb = value == 0 ? 0 : value == 1 ? wd_auto_tstream(ctx) : value;
         return WD_OK;
     case WD_OPT_BLOCK_TILE:
         WD_ARG_CHECK(ctx, value >= 0, "set_option: block tile width must be >= 0");

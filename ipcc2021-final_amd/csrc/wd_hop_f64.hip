@@ -274,6 +274,186 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 1))) 
     }
 }
 
+
+// t-streaming kernel for the both-parity M / M^dagger launches of one system
+// (HOP_M, HOP_M_DOT).  A thread
+// owns the column (x, y, z) for Tb consecutive t.  Site (x, y, z, t) has
+// parity (x + y + z + t) & 1, so the column's next site is the forward t
+// neighbour: the t link of a site serves its forward hop and, as U^dagger,
+// the backward hop of the next site, and the site's input spinor is the
+// backward t neighbour of the next site; both stay in registers (the
+// backward t half spinor of site t + 1 is computed at step t).  The columns
+// of type q, (x + y + z) & 1 == q, give every lane of a wavefront the parity
+// (q + t) & 1 at step t, so a wavefront reads consecutive indices; the two
+// types of a column range are consecutive logical workgroups and read each
+// other's sites as x, y, z neighbours.  Directions are accumulated in the
+// order of k_hop_f64 (x, y, z, t forward, t backward), so out is bit-identical
+// to k_hop_f64's.  HOP_M_DOT partials are per workgroup, indexed by the work
+// unit (t run, column block, type).
+template <bool DAG, int MODE>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_hop_f64_ts(
+    double2 *__restrict__ out, const double2 *__restrict__ in, const double2 *__restrict__ U,
+    Geom g, double a, double hc, const double2 *__restrict__ same,
+    const double2 *__restrict__ dotv, double2 *__restrict__ partials, int Tb, int ncb, int remap,
+    const int *__restrict__ stop)
+{
+    if (stop && *stop)
+        return;
+    const int lb = logical_block(remap);
+    const int q = lb & 1;
+    const int cbk = (lb >> 1) % ncb, k = (lb >> 1) / ncb;
+    const int ncol = g.Lxh * g.Ly * g.Lz; // columns of one type
+    const int c0 = cbk * kBlock + threadIdx.x;
+    // lanes past the last column shadow a valid one and store nothing, so
+    // every thread reaches the workgroup reduction of HOP_M_DOT
+    const bool live = c0 < ncol;
+    const int c = live ? c0 : ncol - 1;
+    const int Lx = 2 * g.Lxh;
+    const int xh = c % g.Lxh;
+    const int y = (c / g.Lxh) % g.Ly;
+    const int z = c / (g.Lxh * g.Ly);
+    const int x = 2 * xh + ((y + z + q) & 1);
+    const int slice = ncol;
+    const int t0 = k * Tb;
+    const int stride = g.stride;
+    const int xf = (x + 1 == Lx) ? 0 : x + 1, xb = (x == 0) ? Lx - 1 : x - 1;
+    const double sxf = x == Lx - 1 ? g.bc[0] : 1.0, sxb = x == 0 ? g.bc[0] : 1.0;
+    const double syf = y == g.Ly - 1 ? g.bc[1] : 1.0, syb = y == 0 ? g.bc[1] : 1.0;
+    const double szf = z == g.Lz - 1 ? g.bc[2] : 1.0, szb = z == 0 ? g.bc[2] : 1.0;
+    const int dyf = (y + 1 == g.Ly) ? -(g.Ly - 1) * g.Lxh : g.Lxh;
+    const int dyb = (y == 0) ? (g.Ly - 1) * g.Lxh : -g.Lxh;
+    const int sz = g.Lxh * g.Ly;
+    const int dzf = (z + 1 == g.Lz) ? -(g.Lz - 1) * sz : sz;
+    const int dzb = (z == 0) ? (g.Lz - 1) * sz : -sz;
+
+    // backward t half spinor of the first site, from (x, y, z, t0 - 1)
+    cpx<double> rc[2][3];
+    {
+        const int tb = (t0 == 0) ? g.Lt - 1 : t0 - 1;
+        const int pb = (q + tb) & 1;
+        const int ib = ((tb * g.Lz + z) * g.Ly + y) * g.Lxh + xh;
+        cpx<double> psi[4][3], W[3][3], h[2][3];
+        load_spinor(in + (long)pb * 12 * stride, stride, ib, psi);
+        load_link<false>(U + (long)((3 * 2 + pb) * 9) * stride, stride, ib, W);
+        using D = Dir<3, false, DAG>;
+        project<D>(psi, h);
+        su3_mul_half<true>(W, h, rc);
+        const double sb = (t0 == 0) ? g.bc[3] : 1.0;
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc)
+                rc[s][cc] = {sb * rc[s][cc].r, sb * rc[s][cc].i};
+    }
+
+    double pr = 0.0, pi = 0.0;
+    for (int t = t0; t < t0 + Tb; ++t) {
+        const int p = (q + t) & 1;
+        const int i = ((t * g.Lz + z) * g.Ly + y) * g.Lxh + xh;
+        const int row = i - xh;
+        const double2 *in_o = in + (long)(1 - p) * 12 * stride;
+        cpx<double> acc[4][3];
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc)
+                acc[s][cc] = {0.0, 0.0};
+        hop_mu<0, DAG, false>(acc, in_o, U, g, p, i, row + (xf >> 1), row + (xb >> 1), sxf, sxb);
+        hop_mu<1, DAG, false>(acc, in_o, U, g, p, i, i + dyf, i + dyb, syf, syb);
+        hop_mu<2, DAG, false>(acc, in_o, U, g, p, i, i + dzf, i + dzb, szf, szb);
+        // forward t with the site's t link, kept for the next site's backward hop
+        cpx<double> W[3][3];
+        load_link<false>(U + (long)((3 * 2 + p) * 9) * stride, stride, i, W);
+        {
+            const int jf = (t + 1 == g.Lt) ? i - (g.Lt - 1) * slice : i + slice;
+            cpx<double> psi[4][3], h[2][3], r[2][3];
+            load_spinor(in_o, stride, jf, psi);
+            using D = Dir<3, true, DAG>;
+            project<D>(psi, h);
+            su3_mul_half<false>(W, h, r);
+            reconstruct<D>(acc, r, t == g.Lt - 1 ? g.bc[3] : 1.0);
+        }
+        pin_acc(acc);
+        {
+            using D = Dir<3, false, DAG>;
+            reconstruct<D>(acc, rc, 1.0);
+        }
+        pin_acc(acc);
+        // the site's input spinor: `same` of HOP_M (when same == in) and the
+        // backward t neighbour of the next site
+        cpx<double> v[4][3];
+        load_spinor(in + (long)p * 12 * stride, stride, i, v);
+        double2 *out_p = out + (long)p * 12 * stride;
+        cpx<double> res[4][3];
+        {
+            const double2 *same_p = same + (long)p * 12 * stride;
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+#pragma unroll
+                for (int cc = 0; cc < 3; ++cc) {
+                    const cpx<double> sv =
+                        (same == in) ? v[s][cc] : ld(same_p, f64_idx(s * 3 + cc, (unsigned)i, 12));
+                    res[s][cc] = {fma(a, sv.r, hc * acc[s][cc].r), fma(a, sv.i, hc * acc[s][cc].i)};
+                }
+        }
+        if (live) {
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+#pragma unroll
+                for (int cc = 0; cc < 3; ++cc)
+                    st(out_p, f64_idx(s * 3 + cc, (unsigned)i, 12), res[s][cc]);
+            if constexpr (MODE == HOP_M_DOT) {
+                const double2 *dv = dotv + (long)p * 12 * stride;
+#pragma unroll
+                for (int s = 0; s < 4; ++s)
+#pragma unroll
+                    for (int cc = 0; cc < 3; ++cc) {
+                        const cpx<double> d = ld(dv, f64_idx(s * 3 + cc, (unsigned)i, 12));
+                        pr = fma(d.r, res[s][cc].r, pr);
+                        pr = fma(d.i, res[s][cc].i, pr);
+                        pi = fma(d.r, res[s][cc].i, pi);
+                        pi = fma(-d.i, res[s][cc].r, pi);
+                    }
+            }
+        }
+        if (t + 1 < t0 + Tb) {
+            cpx<double> h[2][3];
+            using D = Dir<3, false, DAG>;
+            project<D>(v, h);
+            su3_mul_half<true>(W, h, rc);
+        }
+    }
+    if constexpr (MODE == HOP_M_DOT) {
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            pr += __shfl_xor(pr, off, 64);
+            pi += __shfl_xor(pi, off, 64);
+        }
+        __shared__ double sr[kBlock / 64], si[kBlock / 64];
+        const int w = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0) {
+            sr[w] = pr;
+            si[w] = pi;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double tr = 0.0, ti = 0.0;
+            for (int kk = 0; kk < kBlock / 64; ++kk) {
+                tr += sr[kk];
+                ti += si[kk];
+            }
+            partials[lb] = make_double2(tr, ti);
+        }
+    }
+}
+
+// Workgroups of a t-streaming launch with runs of tb.
+static int tstream_grid(const Geom &g, int tb)
+{
+    const int ncb = (g.Lxh * g.Ly * g.Lz + kBlock - 1) / kBlock;
+    return 2 * ncb * (g.Lt / tb);
+}
+
 template <bool DAG>
 int launch_mode(wd_ctx *ctx, double2 *out, const double2 *in, const double2 *U, int cb, int mode,
                 double a, double hc, const double2 *same, const double2 *dotv, double2 *partials,
@@ -286,6 +466,23 @@ int launch_mode(wd_ctx *ctx, double2 *out, const double2 *in, const double2 *U, 
     const int cb_first = (cb < 0) ? 0 : cb;
     if (mode == HOP_M_DOT && partials == ctx->partials && (long)grid * nrhs > ctx->max_partials)
         return wd_set_error(ctx, WD_EARG, "hop: too many workgroups for the partials buffer");
+    if (nparity == 2 && nrhs == 1 && ctx->ts_tb > 0 && !b.mask && !b.a_rhs && !b.c_rhs &&
+        g.Lt % ctx->ts_tb == 0 && (mode == HOP_M || mode == HOP_M_DOT)) {
+        const int tb = ctx->ts_tb;
+        const int ncb = (g.Lxh * g.Ly * g.Lz + kBlock - 1) / kBlock;
+        const int gts = tstream_grid(g, tb);
+        if (mode == HOP_M_DOT && partials == ctx->partials && gts > ctx->max_partials)
+            return wd_set_error(ctx, WD_EARG, "hop: too many workgroups for the partials buffer");
+#define WD_HOP_TS_LAUNCH(M)                                                                        \
+    hipLaunchKernelGGL((k_hop_f64_ts<DAG, M>), dim3(gts), dim3(kBlock), 0, ctx->stream, out, in, U,  \
+                       g, a, hc, same, dotv, partials, tb, ncb, ctx->xcd_remap, stop)
+        if (mode == HOP_M)
+            WD_HOP_TS_LAUNCH(HOP_M);
+        else
+            WD_HOP_TS_LAUNCH(HOP_M_DOT);
+#undef WD_HOP_TS_LAUNCH
+        return wd_check_launch(ctx, "k_hop_f64_ts");
+    }
     const dim3 gd(grid, nrhs);
     const bool il = nparity == 2 && ctx->parity_il;
 #define WD_HOP_F64_LAUNCH(M, I)                                                                    \
@@ -364,8 +561,23 @@ int wd_launch_hop_f64_batch(wd_ctx *ctx, double2 *out, const double2 *in, const 
                                      partials, stop, b, nrhs);
 }
 
-// Number of workgroups a hop launch over `nparity` parities uses (partials count).
+// Number of workgroups a hop launch of one system over `nparity` parities
+// uses (the partials count of HOP_M_DOT).
 int wd_hop_grid(const wd_ctx *ctx, int nparity)
 {
+    if (nparity == 2 && ctx->ts_tb > 0 && ctx->geom.Lt % ctx->ts_tb == 0)
+        return tstream_grid(ctx->geom, ctx->ts_tb);
     return ((ctx->geom.Vh + kBlock - 1) / kBlock) * nparity;
+}
+
+// Automatic t-run length: 16 where the launch keeps at least 512 workgroups
+// (measured on MI355X, fused M: 32^3x64 -7 %, 48^3x96 -4 % against the
+// site-block kernel; with fewer workgroups, e.g. 24^3x48, the site-block
+// kernel is faster).
+int wd_auto_tstream(const wd_ctx *ctx)
+{
+    const Geom &g = ctx->geom;
+    if (g.Lt % 16 != 0)
+        return 0;
+    return tstream_grid(g, 16) >= 512 ? 16 : 0;
 }

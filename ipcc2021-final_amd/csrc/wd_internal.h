@@ -54,6 +54,7 @@ struct wd_ctx {
     size_t staging_bytes;
     int xcd_remap;        // 1: map workgroups so each XCD owns a contiguous site range
     int parity_il;        // 1: both-parity FP64 launches alternate the parities per site block
+    int ts_tb;            // t-run length of the t-streaming both-parity FP64 kernel, 0 = off
     // autotuned (rhs per thread, chunk) of the sloppy operator, keyed by
     // (nrhs, spinor precision, 100 * gauge precision + recon)
     std::map<std::tuple<int, int, int>, std::pair<int, int>> tuned;
@@ -147,6 +148,7 @@ int wd_launch_hop_f64(wd_ctx *ctx, double2 *out, const double2 *in, const double
                       int cb, int mode, double a, const double2 *same, const double2 *dotv,
                       double2 *partials, const int *stop);
 int wd_launch_zero(wd_ctx *ctx, void *p, size_t bytes);
+int wd_auto_tstream(const wd_ctx *ctx);
 int wd_ensure_staging(wd_ctx *ctx, size_t bytes);
 
 // layout / precision conversion (wd_convert.hip)

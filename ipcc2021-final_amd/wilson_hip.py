@@ -31,6 +31,7 @@ OP_HOP, OP_M = 0, 1
 OPT_XCD_REMAP = 1
 OPT_BLOCK_TILE = 2
 OPT_PARITY_ORDER = 3
+OPT_TSTREAM = 4
 RECON_18, RECON_12 = 18, 12
 # autotuner candidates (include/wilson.h WD_TUNE_NR x WD_TUNE_NC)
 TUNE_R = (1, 2, 4)
@@ -46,6 +47,7 @@ EXPORTED = [
     "wd_hop", "wd_diag", "wd_apply_M", "wd_apply_MdagM",
     "wd_norm2", "wd_dot",
     "wd_cg_params_default", "wd_cg", "wd_cg_eo", "wd_time_op", "wd_set_option",
+    "wd_get_option",
     "wd_mixed_params_default", "wd_cg_mixed", "wd_apply_M_sloppy", "wd_tune_report",
     "wd_pion_correlator",
     "wdg_gauge_random_su3", "wdg_gauge_near_identity", "wdg_gauge_unit",
@@ -158,6 +160,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "wd_time_op": ([vp, i, vp, vp, vp, d, i, i, i, ctypes.POINTER(d)], i),
         "wd_pion_correlator": ([vp, vp, i, i, i, dp], i),
         "wd_set_option": ([vp, i, i], i),
+        "wd_get_option": ([vp, i, ctypes.POINTER(ctypes.c_int)], i),
         "wdg_gauge_random_su3": ([ip, ctypes.c_uint64, ctypes.POINTER(dp)], None),
         "wdg_gauge_near_identity": ([ip, ctypes.c_uint64, d, ctypes.POINTER(dp)], None),
         "wdg_gauge_unit": ([ip, ctypes.POINTER(dp)], None),
@@ -383,6 +386,11 @@ class Context:
 
     def set_option(self, option: int, value: int):
         self.check(self.lib.wd_set_option(self.h, option, value), "wd_set_option")
+
+    def get_option(self, option: int) -> int:
+        v = ctypes.c_int()
+        self.check(self.lib.wd_get_option(self.h, option, ctypes.byref(v)), "wd_get_option")
+        return v.value
 
     def norm2(self, f) -> float:
         v = ctypes.c_double()
