@@ -294,12 +294,16 @@ template <bool DAG, int MODE>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_hop_f64_ts(
     double2 *__restrict__ out, const double2 *__restrict__ in, const double2 *__restrict__ U,
     Geom g, double a, double hc, const double2 *__restrict__ same,
-    const double2 *__restrict__ dotv, double2 *__restrict__ partials, int Tb, int ncb, int remap,
-    const int *__restrict__ stop)
+    const double2 *__restrict__ dotv, double2 *__restrict__ partials, int Tb, int ncb, int nwork,
+    int remap, const int *__restrict__ stop)
 {
     if (stop && *stop)
         return;
+    // the grid is padded to a multiple of the XCD count so the XCD mapping
+    // applies (the two column types of a range then share one L2)
     const int lb = logical_block(remap);
+    if (lb >= nwork)
+        return;
     const int q = lb & 1;
     const int cbk = (lb >> 1) % ncb, k = (lb >> 1) / ncb;
     const int ncol = g.Lxh * g.Ly * g.Lz; // columns of one type
@@ -471,11 +475,12 @@ int launch_mode(wd_ctx *ctx, double2 *out, const double2 *in, const double2 *U, 
         const int tb = ctx->ts_tb;
         const int ncb = (g.Lxh * g.Ly * g.Lz + kBlock - 1) / kBlock;
         const int gts = tstream_grid(g, tb);
+        const int glaunch = (gts + kNumXCD - 1) / kNumXCD * kNumXCD;
         if (mode == HOP_M_DOT && partials == ctx->partials && gts > ctx->max_partials)
             return wd_set_error(ctx, WD_EARG, "hop: too many workgroups for the partials buffer");
 #define WD_HOP_TS_LAUNCH(M)                                                                        \
-    hipLaunchKernelGGL((k_hop_f64_ts<DAG, M>), dim3(gts), dim3(kBlock), 0, ctx->stream, out, in, U,  \
-                       g, a, hc, same, dotv, partials, tb, ncb, ctx->xcd_remap, stop)
+    hipLaunchKernelGGL((k_hop_f64_ts<DAG, M>), dim3(glaunch), dim3(kBlock), 0, ctx->stream, out, in, \
+                       U, g, a, hc, same, dotv, partials, tb, ncb, gts, ctx->xcd_remap, stop)
         if (mode == HOP_M)
             WD_HOP_TS_LAUNCH(HOP_M);
         else
@@ -571,9 +576,9 @@ int wd_hop_grid(const wd_ctx *ctx, int nparity)
 }
 
 // Automatic t-run length: 16 where the launch keeps at least 512 workgroups
-// (measured on MI355X, fused M: 32^3x64 -7 %, 48^3x96 -4 % against the
-// site-block kernel; with fewer workgroups, e.g. 24^3x48, the site-block
-// kernel is faster).
+// (measured on MI355X, fused M against the site-block kernel: 32^3x64 -8 %,
+// 40^3x80 -5 %, 48^3x96 -4 %, 64^4 -6 %; with fewer workgroups the
+// site-block kernel is faster: 24^3x48 +19 %, 16^3x32 x3.6).
 int wd_auto_tstream(const wd_ctx *ctx)
 {
     const Geom &g = ctx->geom;
