@@ -198,7 +198,7 @@ typedef struct wd_mixed_report {
     int site_chunk;       /* the L2 chunk the operator ran with */
 } wd_mixed_report;
 
-/* max_iter 100000, tol 1e-10, delta 1e-2, F16 spinors, autotuned rhs per
+/* max_iter 100000, tol 1e-10, delta 2e-2, F16 spinors, autotuned rhs per
  * thread and site chunk, check_every 16, graphs on, full system (even_odd 0),
  * automatic reliable-update slots (reliable_every 0). */
 void wd_mixed_params_default(wd_mixed_params *p);

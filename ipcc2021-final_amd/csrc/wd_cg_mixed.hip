@@ -525,7 +525,9 @@ extern "C" void wd_mixed_params_default(wd_mixed_params *p)
         return;
     p->max_iter = 100000;
     p->tol = 1e-10;
-    p->delta = 1e-2;
+    // measured on MI355X (32^3x64 Schur, F16 spinors): 162 iterations at 1e-2,
+    // 152 at 2e-2, 151 at 3e-2 with one more update; 48 systems within 1 %
+    p->delta = 2e-2;
     p->spinor_prec = WD_F16;
     p->rhs_per_thread = 0;
     p->site_chunk = -1;

@@ -347,7 +347,7 @@ class Context:
                  if ms[ir * TUNE_NC + ic] >= 0}
         return best.value, chunk.value, table
 
-    def cg_mixed(self, x, b, U, Us, masses, tol: float = 1e-10, delta: float = 1e-2,
+    def cg_mixed(self, x, b, U, Us, masses, tol: float = 1e-10, delta: float = 2e-2,
                  spinor_prec: int = F16, rhs_per_thread: int = 0, site_chunk: int = -1,
                  max_iter: int = 100000, check_every: int = 16, use_graph: bool = True,
                  even_odd: bool = False, reliable_every: int = 0):
