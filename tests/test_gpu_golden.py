@@ -5,6 +5,9 @@ DslashoffdNew, DslashNew, DslashEEOONew and CGinvert).
 Bars: operator 1e-13 relative L2; CG the same iteration count as the
 reference's CGinvert from the same rand() start vector and absolute stop
 threshold, residual history equal to the 6 digits the reference prints.
+Both FP64 M kernels are held to these bars: the automatic choice (the
+site-block kernel at these sizes) and the t-streaming kernel forced on
+(the default at 32^3x64 and larger).
 """
 import hashlib
 import json
@@ -28,14 +31,23 @@ def _load(name):
     return np.load(os.path.join(GOLD, MANIFEST["fixtures"][name]["file"]))
 
 
+def _tstream_option(kind, Lt):
+    """1 = automatic; else t runs forced on (the longest of 16, 8, 4, 2 dividing Lt)."""
+    if kind == "auto":
+        return 1
+    return next(T for T in (16, 8, 4, 2) if Lt % T == 0)
+
+
+@pytest.mark.parametrize("kind", ["auto", "tstream"])
 @pytest.mark.parametrize("name", OPS)
-def test_operator_matches_reference_outputs(w, name):
+def test_operator_matches_reference_outputs(w, name, kind):
     f = _load(name)
     L = [int(v) for v in f["L"]]
     A = [np.ascontiguousarray(a) for a in f["gauge"]]
     b = f["src"]
     mass = float(f["mass"])
     with w.Context(L) as ctx:
+        ctx.set_option(w.OPT_TSTREAM, _tstream_option(kind, L[3]))
         U = ctx.gauge(A)
         fin, fout = ctx.field(data=b), ctx.field()
         for dag in (0, 1):
@@ -53,8 +65,9 @@ def test_operator_matches_reference_outputs(w, name):
         assert abs(d - complex(*f["dot"])) < 1e-13 * abs(d)
 
 
+@pytest.mark.parametrize("kind", ["auto", "tstream"])
 @pytest.mark.parametrize("name", CGS)
-def test_cg_matches_reference_iterations(w, name):
+def test_cg_matches_reference_iterations(w, name, kind):
     meta = MANIFEST["fixtures"][name]
     f = _load(name)
     L = meta["lattice"]
@@ -71,12 +84,15 @@ def test_cg_matches_reference_iterations(w, name):
     x0_seed = int(meta["x0"].split("(")[1].split(")")[0])
     x0 = libc_rand_x0(x0_seed, w.volume(L) * 12)
     with w.Context(L) as ctx:
+        ctx.set_option(w.OPT_TSTREAM, _tstream_option(kind, L[3]))
         U = ctx.gauge(A)
         fb, x = ctx.field(data=b), ctx.field(data=x0)
         res = ctx.cg(x, fb, U, meta["mass"], max_iter=meta["max_iter"], tol=meta["acc_abs"],
                      tol_relative=False, use_x0=True)
         xs = x.download()
-    print(f"gpu iterations {res['iterations']} reference {int(f['iterations'])}", flush=True)
+        ctx_t = ctx.get_option(w.OPT_TSTREAM)
+    print(f"gpu iterations {res['iterations']} reference {int(f['iterations'])} "
+          f"(t runs {ctx_t})", flush=True)
     assert res["converged"] and res["iterations"] == int(f["iterations"])
     assert np.allclose(res["history"], f["history"], rtol=1e-5, atol=0)
     assert abs(np.linalg.norm(xs) - float(f["x_norm"])) < 1e-8 * float(f["x_norm"])
