@@ -452,15 +452,18 @@ struct FieldBuf {
 
 // ---- autotuning of the sloppy operator ------------------------------------------
 // Times HOP_M over `nrhs` systems for every (rhs per thread, site chunk)
-// candidate and keeps the fastest per (nrhs, spinor format, link format).
+// candidate and keeps the fastest per (nrhs, spinor format, link format,
+// one or both parities).
 // fixed_R > 0 restricts the search to that rhs-per-thread.
 static const int kTuneR[WD_TUNE_NR] = {1, 2, 4};
 static const int kTuneC[WD_TUNE_NC] = {0, 16, 32, 64};
 
-int wd_tune_sloppy(wd_ctx *ctx, const SloppyArgs &a0, int sprec, int gprec, int recon,
+int wd_tune_sloppy(wd_ctx *ctx, const SloppyArgs &a0, int sprec, int gprec, int recon, int cb,
                    int fixed_R, int *best_R, int *best_C, double *ms_out)
 {
-    const auto key = std::make_tuple(a0.nrhs, sprec, gprec * 100 + recon);
+    // cb = -1: time the both-parity M (full-system solver); cb = 0: the
+    // one-parity launch of the Schur solver, whose best work order differs
+    const auto key = std::make_tuple(a0.nrhs, sprec, gprec * 100 + recon + (cb < 0 ? 0 : 1000));
     auto it = ctx->tuned.find(key);
     if (it != ctx->tuned.end() && !ms_out && (fixed_R <= 0 || it->second.first == fixed_R)) {
         *best_R = it->second.first;
@@ -487,13 +490,13 @@ int wd_tune_sloppy(wd_ctx *ctx, const SloppyArgs &a0, int sprec, int gprec, int 
             SloppyArgs a = a0;
             a.chunk = kTuneC[ic];
             for (int w = 0; w < 2 && !rc; ++w)
-                rc = wd_launch_hop_sloppy(ctx, a, sprec, gprec, recon, R, 0, -1, HOP_M);
+                rc = wd_launch_hop_sloppy(ctx, a, sprec, gprec, recon, R, 0, cb, HOP_M);
             if (rc)
                 break;
             (void)hipEventRecord(e0, ctx->stream);
-            const int reps = 3;
+            const int reps = 5;
             for (int w = 0; w < reps && !rc; ++w)
-                rc = wd_launch_hop_sloppy(ctx, a, sprec, gprec, recon, R, 0, -1, HOP_M);
+                rc = wd_launch_hop_sloppy(ctx, a, sprec, gprec, recon, R, 0, cb, HOP_M);
             (void)hipEventRecord(e1, ctx->stream);
             (void)hipEventSynchronize(e1);
             float ms = 0.f;
@@ -701,7 +704,8 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
         T.in = T.same = (const char *)p.f->data;
         T.in_scale = T.same_scale = p.f->scale;
         int tR = 0, tC = 0;
-        if ((rc = wd_tune_sloppy(ctx, T, sp, Us->prec, Us->recon, R, &tR, &tC, nullptr)))
+        if ((rc = wd_tune_sloppy(ctx, T, sp, Us->prec, Us->recon, eo ? 0 : -1, R, &tR, &tC,
+                                 nullptr)))
             return rc;
         R = tR;
         if (C < 0)
@@ -965,7 +969,7 @@ extern "C" int wd_tune_report(wd_ctx *ctx, const wd_field *f, const wd_gauge *Us
     a.diag = (const float *)diag.p;
     a.nrhs = f->nrhs;
     a.hc = -0.5f;
-    rc = wd_tune_sloppy(ctx, a, f->prec, Us->prec, Us->recon, 0, best_R, best_chunk, ms);
+    rc = wd_tune_sloppy(ctx, a, f->prec, Us->prec, Us->recon, -1, 0, best_R, best_chunk, ms);
     wd_field_destroy(out);
     return rc;
 }
