@@ -212,6 +212,20 @@ int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd_gauge *U,
                 const wd_gauge *Us, const double *masses, const wd_mixed_params *params,
                 wd_mixed_report *reports, double *seconds);
 
+/* Mixed-precision BiCGStab (no counterpart in the reference, whose only
+ * solver is CGinvert): every system is solved on its Schur complement,
+ * A x_e = a b_e + D_eo b_o / 2 with A = a^2 - D_eo D_oe / 4, by BiCGStab with
+ * reliable updates -- two sloppy applications of A per iteration, like the
+ * CGNR's A and A^dag, but with A's condition number instead of A^dag A's --
+ * then x_o = (b_o + D_oe x_e / 2) / a and the same full-system FP64 check of
+ * the reference's stop test |M^dag b - M^dag M x| < tol |M^dag b| as
+ * wd_cg_mixed with even_odd (systems that miss it resume).  Parameters,
+ * reports and return values as wd_cg_mixed (even_odd ignored; iterations
+ * counts BiCGStab iterations). */
+int wd_bicgstab_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd_gauge *U,
+                      const wd_gauge *Us, const double *masses, const wd_mixed_params *params,
+                      wd_mixed_report *reports, double *seconds);
+
 /* out_r = M_r in_r (dagger 0) or M_r^dag in_r with the sloppy operator for all
  * nrhs systems (fields F16 or F32 of the same format, masses[r] host array),
  * rhs_per_thread 1, 2 or 4, site_chunk >= 0 (work order only: every chunk

@@ -425,6 +425,435 @@ __global__ void k_mx_alldone(const int *__restrict__ active, int nrhs, int *__re
     }
 }
 
+
+// ---- mixed BiCGStab on the Schur system ---------------------------------------
+// Per-system state of wd_bicgstab_mixed (complex recursion scalars as (re, im)).
+struct BsState {
+    double2 rho, alpha, omega, beta;
+    double2 r0r;     // (r0, r) of the last reliable update (restarts after a full check)
+    double maxr, target, rsd, true_rsd, b_norm, full_b, full_rsd;
+    int iter, status, updates, pad; // status: 0 running, 1 converged, 2 breakdown, 3 limit
+};
+
+__device__ __forceinline__ double2 zmul(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 zdiv(double2 a, double2 b)
+{
+    const double d = b.x * b.x + b.y * b.y;
+    return make_double2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
+}
+__device__ __forceinline__ bool zfinite(double2 a) { return isfinite(a.x) && isfinite(a.y); }
+
+// common prologue of the elementwise kernels: (parity, site) e < nel of system
+// blockIdx.y; returns false for idle threads
+#define BS_ELEM_PROLOGUE(SFMT)                                                                     \
+    const int rhs = blockIdx.y;                                                                    \
+    const int e = blockIdx.x * kBlock + threadIdx.x;                                              \
+    const int par = e / stride, i = e - par * stride;                                              \
+    const size_t off = rhs * rhs_bytes + (size_t)par * spinor_chunks(SFMT) * 16 * stride;         \
+    const size_t so = ((size_t)rhs * 2 + par) * stride;
+
+// p = r + beta (p - omega v)
+template <int SF>
+__global__ __launch_bounds__(kBlock) void k_bs_p(char *__restrict__ p, float *__restrict__ ps,
+                                                 const char *__restrict__ r,
+                                                 const float *__restrict__ rs,
+                                                 const char *__restrict__ v,
+                                                 const float *__restrict__ vs, size_t rhs_bytes,
+                                                 int stride, int nel, const BsState *__restrict__ st,
+                                                 const int *__restrict__ active)
+{
+    if (active[blockIdx.y] != 1 || blockIdx.x * kBlock + threadIdx.x >= nel)
+        return;
+    BS_ELEM_PROLOGUE(SF)
+    const float br = (float)st[rhs].beta.x, bi = (float)st[rhs].beta.y;
+    const float wr = (float)st[rhs].omega.x, wi = (float)st[rhs].omega.y;
+    cpx<float> vp[4][3], vr[4][3], vv[4][3];
+    Spinor<SF>::template load<float, false>(p + off, ps ? ps + so : nullptr, stride, (unsigned)i, vp);
+    Spinor<SF>::template load<float, false>(r + off, rs ? rs + so : nullptr, stride, (unsigned)i, vr);
+    Spinor<SF>::template load<float, false>(v + off, vs ? vs + so : nullptr, stride, (unsigned)i, vv);
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            // u = p - omega v; p = r + beta u
+            const float ur = vp[s][c].r - (wr * vv[s][c].r - wi * vv[s][c].i);
+            const float ui = vp[s][c].i - (wr * vv[s][c].i + wi * vv[s][c].r);
+            vp[s][c] = {vr[s][c].r + (br * ur - bi * ui), vr[s][c].i + (br * ui + bi * ur)};
+        }
+    Spinor<SF>::template store<float, false>(p + off, ps ? ps + so : nullptr, stride, (unsigned)i, vp);
+}
+
+// s = r - alpha v
+template <int SF>
+__global__ __launch_bounds__(kBlock) void k_bs_s(char *__restrict__ sv, float *__restrict__ ss,
+                                                 const char *__restrict__ r,
+                                                 const float *__restrict__ rs,
+                                                 const char *__restrict__ v,
+                                                 const float *__restrict__ vs, size_t rhs_bytes,
+                                                 int stride, int nel, const BsState *__restrict__ st,
+                                                 const int *__restrict__ active)
+{
+    if (active[blockIdx.y] != 1 || blockIdx.x * kBlock + threadIdx.x >= nel)
+        return;
+    BS_ELEM_PROLOGUE(SF)
+    const float ar = (float)st[rhs].alpha.x, ai = (float)st[rhs].alpha.y;
+    cpx<float> vr[4][3], vv[4][3];
+    Spinor<SF>::template load<float, false>(r + off, rs ? rs + so : nullptr, stride, (unsigned)i, vr);
+    Spinor<SF>::template load<float, false>(v + off, vs ? vs + so : nullptr, stride, (unsigned)i, vv);
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            vr[s][c] = {vr[s][c].r - (ar * vv[s][c].r - ai * vv[s][c].i),
+                        vr[s][c].i - (ar * vv[s][c].i + ai * vv[s][c].r)};
+    Spinor<SF>::template store<float, false>(sv + off, ss ? ss + so : nullptr, stride, (unsigned)i, vr);
+}
+
+// partial |t|^2 per (system, workgroup)
+template <int SF>
+__global__ __launch_bounds__(kBlock) void k_bs_norm(const char *__restrict__ t,
+                                                    const float *__restrict__ ts, size_t rhs_bytes,
+                                                    int stride, int nel,
+                                                    const int *__restrict__ active,
+                                                    double2 *__restrict__ partials)
+{
+    const int rhs = blockIdx.y;
+    if (active[rhs] != 1)
+        return;
+    double acc = 0.0;
+    if (blockIdx.x * kBlock + threadIdx.x < nel) {
+        BS_ELEM_PROLOGUE(SF)
+        cpx<float> vt[4][3];
+        Spinor<SF>::template load<float, false>(t + off, ts ? ts + so : nullptr, stride, (unsigned)i,
+                                               vt);
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                acc = fma((double)vt[s][c].r, (double)vt[s][c].r, acc);
+                acc = fma((double)vt[s][c].i, (double)vt[s][c].i, acc);
+            }
+    }
+    block_reduce_store(acc, 0.0, partials + (size_t)rhs * gridDim.x + blockIdx.x);
+}
+
+// x_s += alpha p + omega s; r = s - omega t; partials |r|^2 (pa) and (r0, r) (pb)
+// of the stored r
+template <int SF>
+__global__ __launch_bounds__(kBlock) void k_bs_update(
+    char *__restrict__ xs, char *__restrict__ r, float *__restrict__ rs, const char *__restrict__ p,
+    const float *__restrict__ ps, const char *__restrict__ sv, const float *__restrict__ ss,
+    const char *__restrict__ t, const float *__restrict__ ts, const char *__restrict__ r0,
+    const float *__restrict__ r0s, size_t rhs_bytes, size_t xs_rhs_bytes, int stride, int nel,
+    const BsState *__restrict__ st, const int *__restrict__ active, double2 *__restrict__ pa,
+    double2 *__restrict__ pb)
+{
+    const int rhs = blockIdx.y;
+    if (active[rhs] != 1)
+        return;
+    double rr = 0.0, dr = 0.0, di = 0.0;
+    if (blockIdx.x * kBlock + threadIdx.x < nel) {
+        BS_ELEM_PROLOGUE(SF)
+        const size_t xoff = rhs * xs_rhs_bytes + (size_t)par * spinor_chunks(WD_F32) * 16 * stride;
+        const float ar = (float)st[rhs].alpha.x, ai = (float)st[rhs].alpha.y;
+        const float wr = (float)st[rhs].omega.x, wi = (float)st[rhs].omega.y;
+        cpx<float> vx[4][3], vp[4][3], vs[4][3], vt[4][3];
+        Spinor<WD_F32>::template load<float, false>(xs + xoff, nullptr, stride, (unsigned)i, vx);
+        Spinor<SF>::template load<float, false>(p + off, ps ? ps + so : nullptr, stride, (unsigned)i,
+                                               vp);
+        Spinor<SF>::template load<float, false>(sv + off, ss ? ss + so : nullptr, stride,
+                                               (unsigned)i, vs);
+        Spinor<SF>::template load<float, false>(t + off, ts ? ts + so : nullptr, stride, (unsigned)i,
+                                               vt);
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const cpx<float> P = vp[s][c], S = vs[s][c], T = vt[s][c];
+                vx[s][c] = {vx[s][c].r + (ar * P.r - ai * P.i) + (wr * S.r - wi * S.i),
+                            vx[s][c].i + (ar * P.i + ai * P.r) + (wr * S.i + wi * S.r)};
+                vs[s][c] = {S.r - (wr * T.r - wi * T.i), S.i - (wr * T.i + wi * T.r)};
+            }
+        Spinor<WD_F32>::template store<float, false>(xs + xoff, nullptr, stride, (unsigned)i, vx);
+        cpx<float> vq[4][3]; // r as stored
+        if constexpr (SF == WD_F16) {
+            unsigned wv[12];
+            const float sc = Spinor<WD_F16>::template store_packed<float, false>(
+                r + off, rs + so, stride, (unsigned)i, vs, wv);
+#pragma unroll
+            for (int k = 0; k < 12; ++k)
+                vq[k / 3][k % 3] = h2f(wv[k], sc);
+        } else {
+            Spinor<SF>::template store<float, false>(r + off, nullptr, stride, (unsigned)i, vs);
+#pragma unroll
+            for (int k = 0; k < 12; ++k)
+                vq[k / 3][k % 3] = vs[k / 3][k % 3];
+        }
+        cpx<float> v0[4][3];
+        Spinor<SF>::template load<float, false>(r0 + off, r0s ? r0s + so : nullptr, stride,
+                                               (unsigned)i, v0);
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double qr = vq[s][c].r, qi = vq[s][c].i, zr = v0[s][c].r, zi = v0[s][c].i;
+                rr = fma(qr, qr, fma(qi, qi, rr));
+                dr = fma(zr, qr, fma(zi, qi, dr));
+                di = fma(zr, qi, fma(-zi, qr, di));
+            }
+    }
+    block_reduce_store(rr, 0.0, pa + (size_t)rhs * gridDim.x + blockIdx.x);
+    block_reduce_store(dr, di, pb + (size_t)rhs * gridDim.x + blockIdx.x);
+}
+
+// r = bp - q64 (F64; q64 == nullptr: r = bp) -> sloppy r (and r0 = r when r0w);
+// partials |r|^2 (pa, FP64 values) and (r0, r) (pb, stored values); systems
+// with mask[r] only when mask != nullptr
+template <int SF>
+__global__ __launch_bounds__(kBlock) void k_bs_relr(
+    char *__restrict__ r, float *__restrict__ rs, char *__restrict__ r0, float *__restrict__ r0s,
+    bool r0w, const double2 *__restrict__ bp, const double2 *__restrict__ q64, size_t x_rhs_elems,
+    size_t rhs_bytes, int stride, int nel, const int *__restrict__ mask, double2 *__restrict__ pa,
+    double2 *__restrict__ pb)
+{
+    const int rhs = blockIdx.y;
+    if (mask && !mask[rhs])
+        return;
+    double rr = 0.0, dr = 0.0, di = 0.0;
+    if (blockIdx.x * kBlock + threadIdx.x < nel) {
+        BS_ELEM_PROLOGUE(SF)
+        const size_t xo = rhs * x_rhs_elems + (size_t)par * 12 * stride;
+        cpx<double> vb[4][3];
+        Spinor<WD_F64>::load(bp + xo, nullptr, stride, (unsigned)i, vb);
+        if (q64) {
+            cpx<double> vq[4][3];
+            Spinor<WD_F64>::load(q64 + xo, nullptr, stride, (unsigned)i, vq);
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    vb[s][c] = {vb[s][c].r - vq[s][c].r, vb[s][c].i - vq[s][c].i};
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                rr = fma(vb[s][c].r, vb[s][c].r, rr);
+                rr = fma(vb[s][c].i, vb[s][c].i, rr);
+            }
+        Spinor<SF>::store(r + off, rs ? rs + so : nullptr, stride, (unsigned)i, vb);
+        if (r0w)
+            Spinor<SF>::store(r0 + off, r0s ? r0s + so : nullptr, stride, (unsigned)i, vb);
+        cpx<float> vq[4][3], v0[4][3];
+        Spinor<SF>::template load<float, false>(r + off, rs ? rs + so : nullptr, stride, (unsigned)i,
+                                               vq);
+        Spinor<SF>::template load<float, false>(r0 + off, r0s ? r0s + so : nullptr, stride,
+                                               (unsigned)i, v0);
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double qr = vq[s][c].r, qi = vq[s][c].i, zr = v0[s][c].r, zi = v0[s][c].i;
+                dr = fma(zr, qr, fma(zi, qi, dr));
+                di = fma(zr, qi, fma(-zi, qr, di));
+            }
+    }
+    block_reduce_store(rr, 0.0, pa + (size_t)rhs * gridDim.x + blockIdx.x);
+    block_reduce_store(dr, di, pb + (size_t)rhs * gridDim.x + blockIdx.x);
+}
+#undef BS_ELEM_PROLOGUE
+
+// ---- per-system scalar kernels (blockIdx.x = system, 1024 threads) ----
+__device__ __forceinline__ double2 sum_at(const double2 *partials, int count)
+{
+    return sum_partials_1024(partials + (size_t)blockIdx.x * count, count);
+}
+
+__global__ __launch_bounds__(1024) void k_bs_init(const double2 *__restrict__ pa,
+                                                  const double2 *__restrict__ pb, int count,
+                                                  BsState *__restrict__ st, int *__restrict__ active,
+                                                  int *__restrict__ reliable, double tol)
+{
+    const double2 rr = sum_at(pa, count);
+    const double2 r0r = sum_at(pb, count);
+    if (threadIdx.x == 0) {
+        const int r = blockIdx.x;
+        BsState m = {};
+        m.b_norm = sqrt(rr.x);
+        m.maxr = m.b_norm;
+        m.rsd = m.true_rsd = m.b_norm;
+        m.target = tol * m.b_norm;
+        m.rho = m.r0r = r0r;
+        m.alpha = m.omega = make_double2(1.0, 0.0);
+        m.beta = make_double2(0.0, 0.0);
+        reliable[r] = 0;
+        if (!(m.b_norm > 0.0)) { // x = 0 solves b = 0
+            m.status = 1;
+            active[r] = 0;
+        } else {
+            active[r] = 1;
+        }
+        st[r] = m;
+    }
+}
+
+// alpha = rho / (r0, v)
+__global__ __launch_bounds__(1024) void k_bs_alpha(const double2 *__restrict__ pa, int count,
+                                                   BsState *__restrict__ st,
+                                                   int *__restrict__ active)
+{
+    const int r = blockIdx.x;
+    if (active[r] != 1)
+        return;
+    const double2 d = sum_at(pa, count);
+    if (threadIdx.x == 0) {
+        if (!zfinite(d) || !(d.x * d.x + d.y * d.y > 0.0)) {
+            st[r].status = 2;
+            active[r] = 0;
+            return;
+        }
+        st[r].alpha = zdiv(st[r].rho, d);
+    }
+}
+
+// omega = (t, s) / (t, t) = conj((s, t)) / |t|^2; t = 0 (s = 0: the system is
+// solved) gives omega = 0
+__global__ __launch_bounds__(1024) void k_bs_omega(const double2 *__restrict__ pst, int cst,
+                                                   const double2 *__restrict__ ptt, int ctt,
+                                                   BsState *__restrict__ st,
+                                                   int *__restrict__ active)
+{
+    const int r = blockIdx.x;
+    if (active[r] != 1)
+        return;
+    const double2 s_t = sum_at(pst, cst);
+    const double2 tt = sum_at(ptt, ctt);
+    if (threadIdx.x == 0) {
+        if (!zfinite(s_t) || !isfinite(tt.x)) {
+            st[r].status = 2;
+            active[r] = 0;
+            return;
+        }
+        st[r].omega = tt.x > 0.0 ? make_double2(s_t.x / tt.x, -s_t.y / tt.x)
+                                 : make_double2(0.0, 0.0);
+    }
+}
+
+// beta = (rho_new / rho) (alpha / omega); restart direction (beta = 0) when
+// omega or rho vanished
+__device__ __forceinline__ void bs_next(BsState &m, double2 r0r)
+{
+    const bool ok = (m.omega.x != 0.0 || m.omega.y != 0.0) && (m.rho.x != 0.0 || m.rho.y != 0.0);
+    m.beta = ok ? zmul(zdiv(r0r, m.rho), zdiv(m.alpha, m.omega)) : make_double2(0.0, 0.0);
+    m.rho = r0r;
+}
+
+__global__ __launch_bounds__(1024) void k_bs_decide(const double2 *__restrict__ pa,
+                                                    const double2 *__restrict__ pb, int count,
+                                                    BsState *__restrict__ st,
+                                                    int *__restrict__ active,
+                                                    int *__restrict__ reliable, int max_iter,
+                                                    double delta)
+{
+    const int r = blockIdx.x;
+    if (active[r] != 1)
+        return;
+    const double2 rr = sum_at(pa, count);
+    const double2 r0r = sum_at(pb, count);
+    if (threadIdx.x == 0) {
+        BsState &m = st[r];
+        const double rsd = sqrt(rr.x);
+        m.iter += 1;
+        m.rsd = rsd;
+        if (rsd < delta * m.maxr || rsd < m.target || m.iter >= max_iter || !isfinite(rr.x) ||
+            !zfinite(r0r)) {
+            reliable[r] = 1;
+            active[r] = 2; // waits for the next reliable-update slot
+        } else {
+            bs_next(m, r0r);
+        }
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_bs_relfin(const double2 *__restrict__ pa,
+                                                    const double2 *__restrict__ pb, int count,
+                                                    BsState *__restrict__ st,
+                                                    int *__restrict__ active,
+                                                    int *__restrict__ reliable, int max_iter)
+{
+    const int r = blockIdx.x;
+    if (!reliable[r])
+        return;
+    const double2 rr = sum_at(pa, count);
+    const double2 r0r = sum_at(pb, count);
+    if (threadIdx.x == 0) {
+        BsState &m = st[r];
+        const double rsd = sqrt(rr.x);
+        m.true_rsd = rsd;
+        m.updates += 1;
+        m.r0r = r0r;
+        reliable[r] = 0;
+        if (rsd < m.target) {
+            m.status = 1;
+            active[r] = 0;
+        } else if (m.iter >= max_iter || !isfinite(rr.x) || !zfinite(r0r)) {
+            m.status = isfinite(rr.x) ? 3 : 2;
+            active[r] = 0;
+        } else {
+            bs_next(m, r0r);
+            m.maxr = rsd;
+            active[r] = 1;
+        }
+    }
+}
+
+__global__ void k_bs_markconv(const BsState *__restrict__ st, int nrhs, int *__restrict__ check)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nrhs)
+        check[r] = st[r].status == 1 ? 1 : 0;
+}
+
+__global__ __launch_bounds__(1024) void k_bs_fullb(const double2 *__restrict__ partials, int count,
+                                                   BsState *__restrict__ st)
+{
+    const double2 s = sum_at(partials, count);
+    if (threadIdx.x == 0)
+        st[blockIdx.x].full_b = sqrt(s.x);
+}
+
+// Systems that met the Schur target: accept when the full-system residual
+// meets tol, else resume with a tighter target, restarting the direction
+// (p = r) from the true Schur residual of the last reliable update.
+__global__ __launch_bounds__(1024) void k_bs_fullfin(const double2 *__restrict__ partials,
+                                                     int count, BsState *__restrict__ st,
+                                                     int *__restrict__ active,
+                                                     int *__restrict__ check, double tol,
+                                                     int max_iter)
+{
+    const int r = blockIdx.x;
+    if (!check[r])
+        return;
+    const double2 s = sum_at(partials, count);
+    if (threadIdx.x == 0) {
+        BsState &m = st[r];
+        const double full = sqrt(s.x);
+        m.full_rsd = full;
+        check[r] = 0;
+        if (full <= tol * m.full_b || !(m.full_b > 0.0) || m.iter >= max_iter || !isfinite(full))
+            return;
+        m.target *= 0.5 * tol * m.full_b / full;
+        m.rho = m.r0r;
+        m.beta = make_double2(0.0, 0.0);
+        m.maxr = m.true_rsd;
+        m.status = 0;
+        active[r] = 1;
+    }
+}
+
 struct DevBuf {
     void *p = nullptr;
     ~DevBuf()
@@ -939,6 +1368,368 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
     }
     if (breakdown)
         return wd_set_error(ctx, WD_EBREAKDOWN, "mixed cg: breakdown in at least one system");
+    return all_conv ? WD_OK : WD_ENOTCONV;
+}
+
+
+// Mixed-precision BiCGStab (van der Vorst 1992) with reliable updates
+// (Sleijpen & van der Vorst 1996) on the Schur system A x_e = bh of every
+// system, A = a^2 - D_eo D_oe / 4, bh = a b_e + D_eo b_o / 2: two
+// applications of A per iteration, as CGNR's A and A^dag, but A's condition
+// number instead of A^dag A's.  Same sloppy operator, FP64 reliable updates
+// (r = bh - A x) in slots, and the same full-system FP64 check of the
+// reference's stop test as wd_cg_mixed with even_odd.
+extern "C" int wd_bicgstab_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd_gauge *U,
+                                 const wd_gauge *Us, const double *masses,
+                                 const wd_mixed_params *prm, wd_mixed_report *reports,
+                                 double *seconds)
+{
+    if (!ctx)
+        return WD_EARG;
+    WD_ARG_CHECK(ctx, x && b && U && Us && masses && prm, "mixed bicgstab: null argument");
+    WD_ARG_CHECK(ctx, x != b, "mixed bicgstab: x and b must be distinct fields");
+    WD_ARG_CHECK(ctx, x->prec == WD_F64 && b->prec == WD_F64, "mixed bicgstab: x and b must be F64");
+    WD_ARG_CHECK(ctx, x->nrhs == b->nrhs, "mixed bicgstab: x and b nrhs differ");
+    WD_ARG_CHECK(ctx, U->prec == WD_F64 && U->recon == WD_RECON_18,
+                 "mixed bicgstab: U must be F64 18-real (reliable updates)");
+    WD_ARG_CHECK(ctx, prm->spinor_prec == WD_F16 || prm->spinor_prec == WD_F32,
+                 "mixed bicgstab: spinor_prec must be F16 or F32");
+    WD_ARG_CHECK(ctx, wd_sloppy_supported(prm->spinor_prec, Us->prec, Us->recon),
+                 "mixed bicgstab: unsupported sloppy link format");
+    WD_ARG_CHECK(ctx, prm->rhs_per_thread == 0 || prm->rhs_per_thread == 1 ||
+                          prm->rhs_per_thread == 2 || prm->rhs_per_thread == 4,
+                 "mixed bicgstab: rhs_per_thread must be 0, 1, 2 or 4");
+    WD_ARG_CHECK(ctx, prm->site_chunk >= -1, "mixed bicgstab: site_chunk must be >= -1");
+    WD_ARG_CHECK(ctx, prm->max_iter >= 1 && prm->check_every >= 1 && prm->tol > 0.0 &&
+                          prm->delta > 0.0 && prm->delta < 1.0 && prm->reliable_every >= 0,
+                 "mixed bicgstab: bad iteration / tolerance parameters");
+    const auto t0 = std::chrono::steady_clock::now();
+    const int n = b->nrhs;
+    const int sp = prm->spinor_prec;
+    const int stride = (int)ctx->stride;
+    int rc;
+
+    // F64: bh (even half), A x, M^dag b and the check's M^dag M x; sloppy: r, r0,
+    // p, v, s, t (the odd halves of v and t hold the operator's intermediate hop)
+    FieldBuf bh, Mfull, t64, q64, r, r0, pv, v, sv, t, xs;
+    if ((rc = wd_field_create(ctx, &bh.f, WD_F64, n)) || (rc = wd_field_create(ctx, &t64.f, WD_F64, n)) ||
+        (rc = wd_field_create(ctx, &q64.f, WD_F64, n)) || (rc = wd_field_create(ctx, &Mfull.f, WD_F64, n)) ||
+        (rc = wd_field_create(ctx, &r.f, sp, n)) || (rc = wd_field_create(ctx, &r0.f, sp, n)) ||
+        (rc = wd_field_create(ctx, &pv.f, sp, n)) || (rc = wd_field_create(ctx, &v.f, sp, n)) ||
+        (rc = wd_field_create(ctx, &sv.f, sp, n)) || (rc = wd_field_create(ctx, &t.f, sp, n)) ||
+        (rc = wd_field_create(ctx, &xs.f, WD_F32, n)))
+        return rc;
+    const int eg_full = (2 * stride + kBlock - 1) / kBlock;
+    const int nel = stride; // the even half
+    const int eg = (nel + kBlock - 1) / kBlock;
+    const int bpp = (int)((ctx->Vh + kBlock - 1) / kBlock);
+    const int hop_nsb = bpp; // partials of a one-parity operator launch
+    const int nsb_max = std::max(eg_full, 2 * bpp);
+    DevBuf pa_buf, pb_buf, state, flags, diag_d, diag_f, coef;
+    if ((rc = pa_buf.alloc(ctx, sizeof(double2) * (size_t)n * nsb_max)) ||
+        (rc = pb_buf.alloc(ctx, sizeof(double2) * (size_t)n * nsb_max)) ||
+        (rc = state.alloc(ctx, sizeof(BsState) * n)) ||
+        (rc = flags.alloc(ctx, sizeof(int) * (3 * (size_t)n + 4))) ||
+        (rc = diag_d.alloc(ctx, sizeof(double) * n)) || (rc = diag_f.alloc(ctx, sizeof(float) * n)) ||
+        (rc = coef.alloc(ctx, sizeof(double) * 3 * n)))
+        return rc;
+    BsState *st = (BsState *)state.p;
+    int *active = (int *)flags.p, *reliable = active + n, *check = reliable + n, *stop = check + n;
+    double2 *pa = (double2 *)pa_buf.p, *pb = (double2 *)pb_buf.p;
+    double *a2_d = (double *)coef.p, *inva_d = a2_d + n, *inv2a_d = inva_d + n;
+    {
+        std::vector<double> ad(n), cd(3 * (size_t)n);
+        std::vector<float> af(n);
+        for (int k = 0; k < n; ++k) {
+            ad[k] = 4.0 + masses[k];
+            af[k] = (float)(ad[k] * ad[k]);
+            cd[k] = ad[k] * ad[k];
+            cd[n + k] = 1.0 / ad[k];
+            cd[2 * n + k] = 0.5 / ad[k];
+        }
+        WD_HIP_CHECK(ctx, hipMemcpyAsync(diag_d.p, ad.data(), sizeof(double) * n,
+                                         hipMemcpyHostToDevice, ctx->stream));
+        WD_HIP_CHECK(ctx, hipMemcpyAsync(diag_f.p, af.data(), sizeof(float) * n,
+                                         hipMemcpyHostToDevice, ctx->stream));
+        WD_HIP_CHECK(ctx, hipMemcpyAsync(coef.p, cd.data(), sizeof(double) * 3 * n,
+                                         hipMemcpyHostToDevice, ctx->stream));
+        WD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    const double2 *Ud = (const double2 *)U->data;
+    const size_t xe = x->rhs_elems;
+    const size_t srb = r.f->rhs_elems * prec_bytes_complex(sp);
+    const size_t xsb = xs.f->rhs_elems * prec_bytes_complex(WD_F32);
+    const double *ddiag = (const double *)diag_d.p;
+    F64Batch all = {xe, nullptr, ddiag, nullptr};
+    F64Batch rel = {xe, reliable, ddiag, nullptr};
+    F64Batch rel2 = {xe, reliable, a2_d, nullptr};
+    double2 *xd = (double2 *)x->data;
+    const double2 *bd = (const double2 *)b->data;
+    double2 *bhd = (double2 *)bh.f->data, *t64d = (double2 *)t64.f->data,
+            *q64d = (double2 *)q64.f->data;
+
+    // bh = a b_e + D_eo b_o / 2; M^dag b for the full-system check
+    if ((rc = wd_launch_hop_f64_batch_c(ctx, bhd, bd, Ud, 0, 0, HOP_M, 0.5, bd, nullptr, nullptr,
+                                        nullptr, all, n)) ||
+        (rc = wd_launch_hop_f64_batch(ctx, (double2 *)Mfull.f->data, bd, Ud, 1, -1, HOP_M, bd,
+                                      nullptr, nullptr, nullptr, all, n)) ||
+        (rc = wd_launch_zero(ctx, x->data, x->bytes)) ||
+        (rc = wd_launch_zero(ctx, pv.f->data, pv.f->bytes)) ||
+        (rc = wd_launch_zero(ctx, v.f->data, v.f->bytes)) ||
+        (rc = wd_launch_zero(ctx, xs.f->data, xs.f->bytes)))
+        return rc;
+    // r = r0 = bh, rho = (r0, r)
+    if (sp == WD_F16)
+        hipLaunchKernelGGL(k_bs_relr<WD_F16>, dim3(eg, n), dim3(kBlock), 0, ctx->stream,
+                           (char *)r.f->data, r.f->scale, (char *)r0.f->data, r0.f->scale, true,
+                           (const double2 *)bhd, (const double2 *)nullptr, xe, srb, stride, nel,
+                           (const int *)nullptr, pa, pb);
+    else
+        hipLaunchKernelGGL(k_bs_relr<WD_F32>, dim3(eg, n), dim3(kBlock), 0, ctx->stream,
+                           (char *)r.f->data, r.f->scale, (char *)r0.f->data, r0.f->scale, true,
+                           (const double2 *)bhd, (const double2 *)nullptr, xe, srb, stride, nel,
+                           (const int *)nullptr, pa, pb);
+    // the Schur target starts at half the full-system tolerance (relative to |bh|)
+    hipLaunchKernelGGL(k_bs_init, dim3(n), dim3(1024), 0, ctx->stream, (const double2 *)pa,
+                       (const double2 *)pb, eg, st, active, reliable, 0.5 * prm->tol);
+    hipLaunchKernelGGL(k_mx_fullres, dim3(eg_full, n), dim3(kBlock), 0, ctx->stream,
+                       (const double2 *)Mfull.f->data, (const double2 *)nullptr, xe, stride,
+                       (const int *)nullptr, pa);
+    hipLaunchKernelGGL(k_bs_fullb, dim3(n), dim3(1024), 0, ctx->stream, (const double2 *)pa,
+                       eg_full, st);
+    hipLaunchKernelGGL(k_mx_alldone, dim3(1), dim3(64), 0, ctx->stream, active, n, stop);
+    if ((rc = wd_check_launch(ctx, "mixed bicgstab init")))
+        return rc;
+
+    // sloppy A (two one-parity launches), with the (dotv, out) partials
+    SloppyArgs As = {};
+    As.rhs_bytes = srb;
+    As.U = (const char *)Us->data;
+    As.diag = (const float *)diag_f.p;
+    As.active = active;
+    As.partials = pa;
+    As.stop = stop;
+    As.nrhs = n;
+    As.hc = -0.5f;
+    int R = prm->rhs_per_thread;
+    int C = prm->site_chunk;
+    if (R == 0 || C < 0) {
+        SloppyArgs T = As;
+        T.out = (char *)v.f->data;
+        T.out_scale = v.f->scale;
+        T.in = T.same = (const char *)pv.f->data;
+        T.in_scale = T.same_scale = pv.f->scale;
+        int tR = 0, tC = 0;
+        if ((rc = wd_tune_sloppy(ctx, T, sp, Us->prec, Us->recon, 0, R, &tR, &tC, nullptr)))
+            return rc;
+        R = tR;
+        if (C < 0)
+            C = tC;
+    }
+    As.chunk = C;
+    auto apply_A = [&](wd_field *out, const wd_field *in, const wd_field *dotv) -> int {
+        SloppyArgs m = As;
+        m.out = (char *)out->data;
+        m.out_scale = out->scale;
+        m.in = m.same = (const char *)in->data;
+        m.in_scale = m.same_scale = in->scale;
+        m.dotv = (const char *)dotv->data;
+        m.dotv_scale = dotv->scale;
+        SloppyArgs h = m; // +D_oe in / 2 into the odd half of out
+        h.hc = 0.5f;
+        int e = wd_launch_hop_sloppy(ctx, h, sp, Us->prec, Us->recon, R, 0, 1, HOP_ONLY);
+        if (e)
+            return e;
+        m.in = (const char *)out->data; // a^2 in_e - D_eo (odd half of out) / 2
+        m.in_scale = out->scale;
+        return wd_launch_hop_sloppy(ctx, m, sp, Us->prec, Us->recon, R, 0, 0, HOP_M_DOT);
+    };
+    // FP64 A v on the even half (the odd half of out holds +D_oe v / 2)
+    auto apply_A64 = [&](const double2 *in, double2 *out, const F64Batch &b1, const F64Batch &b2,
+                         const int *stp) -> int {
+        int e = wd_launch_hop_f64_batch_c(ctx, out, in, Ud, 0, 1, HOP_ONLY, 0.5, nullptr, nullptr,
+                                          nullptr, stp, b1, n);
+        if (e)
+            return e;
+        return wd_launch_hop_f64_batch_c(ctx, out, out, Ud, 0, 0, HOP_M, -0.5, in, nullptr, nullptr,
+                                         stp, b2, n);
+    };
+
+#define BS_SF(K, ...)                                                                              \
+    do {                                                                                           \
+        if (sp == WD_F16)                                                                          \
+            hipLaunchKernelGGL(K<WD_F16>, dim3(eg, n), dim3(kBlock), 0, ctx->stream, __VA_ARGS__);  \
+        else                                                                                       \
+            hipLaunchKernelGGL(K<WD_F32>, dim3(eg, n), dim3(kBlock), 0, ctx->stream, __VA_ARGS__);  \
+    } while (0)
+
+    auto iteration = [&]() -> int {
+        int e;
+        BS_SF(k_bs_p, (char *)pv.f->data, pv.f->scale, (const char *)r.f->data,
+              (const float *)r.f->scale, (const char *)v.f->data, (const float *)v.f->scale, srb,
+              stride, nel, (const BsState *)st, (const int *)active);
+        if ((e = apply_A(v.f, pv.f, r0.f)))
+            return e;
+        hipLaunchKernelGGL(k_bs_alpha, dim3(n), dim3(1024), 0, ctx->stream, (const double2 *)pa,
+                           hop_nsb, st, active);
+        BS_SF(k_bs_s, (char *)sv.f->data, sv.f->scale, (const char *)r.f->data,
+              (const float *)r.f->scale, (const char *)v.f->data, (const float *)v.f->scale, srb,
+              stride, nel, (const BsState *)st, (const int *)active);
+        if ((e = apply_A(t.f, sv.f, sv.f)))
+            return e;
+        BS_SF(k_bs_norm, (const char *)t.f->data, (const float *)t.f->scale, srb, stride, nel,
+              (const int *)active, pb);
+        hipLaunchKernelGGL(k_bs_omega, dim3(n), dim3(1024), 0, ctx->stream, (const double2 *)pa,
+                           hop_nsb, (const double2 *)pb, eg, st, active);
+        BS_SF(k_bs_update, (char *)xs.f->data, (char *)r.f->data, r.f->scale,
+              (const char *)pv.f->data, (const float *)pv.f->scale, (const char *)sv.f->data,
+              (const float *)sv.f->scale, (const char *)t.f->data, (const float *)t.f->scale,
+              (const char *)r0.f->data, (const float *)r0.f->scale, srb, xsb, stride, nel,
+              (const BsState *)st, (const int *)active, pa, pb);
+        hipLaunchKernelGGL(k_bs_decide, dim3(n), dim3(1024), 0, ctx->stream, (const double2 *)pa,
+                           (const double2 *)pb, eg, st, active, reliable, prm->max_iter,
+                           prm->delta);
+        return wd_check_launch(ctx, "mixed bicgstab iteration");
+    };
+
+    // reliable update of the waiting systems: x += x_s, r = bh - A x (FP64)
+    auto reliable_slot = [&]() -> int {
+        int e;
+        hipLaunchKernelGGL(k_mx_relx, dim3(eg, n), dim3(kBlock), 0, ctx->stream, xd,
+                           (char *)xs.f->data, xe, xsb, stride, nel, (const int *)reliable);
+        if ((e = wd_check_launch(ctx, "k_mx_relx")) ||
+            (e = apply_A64(xd, t64d, rel, rel2, stop)))
+            return e;
+        BS_SF(k_bs_relr, (char *)r.f->data, r.f->scale, (char *)r0.f->data, r0.f->scale, false,
+              (const double2 *)bhd, (const double2 *)t64d, xe, srb, stride, nel,
+              (const int *)reliable, pa, pb);
+        hipLaunchKernelGGL(k_bs_relfin, dim3(n), dim3(1024), 0, ctx->stream, (const double2 *)pa,
+                           (const double2 *)pb, eg, st, active, reliable, prm->max_iter);
+        hipLaunchKernelGGL(k_mx_alldone, dim3(1), dim3(64), 0, ctx->stream, (const int *)active,
+                           n, stop);
+        return wd_check_launch(ctx, "mixed bicgstab reliable update");
+    };
+#undef BS_SF
+
+    const int chunk = prm->check_every;
+    const int slot_every = prm->reliable_every > 0 ? prm->reliable_every : (n == 1 ? 8 : 4);
+    auto block = [&]() -> int {
+        int e = 0;
+        for (int k = 0; k < chunk && !e; ++k) {
+            e = iteration();
+            if (!e && ((k + 1) % slot_every == 0 || k + 1 == chunk))
+                e = reliable_slot();
+        }
+        return e;
+    };
+
+    // x_o = b_o / a + D_oe x_e / (2a) and |M^dag b - M^dag M x| for the systems
+    // that met the Schur target; the ones that miss tol resume
+    auto full_check = [&]() -> int {
+        int e;
+        F64Batch chk_rec = {xe, check, inva_d, inv2a_d};
+        F64Batch chk = {xe, check, ddiag, nullptr};
+        hipLaunchKernelGGL(k_bs_markconv, dim3((n + 255) / 256), dim3(256), 0, ctx->stream,
+                           (const BsState *)st, n, check);
+        if ((e = wd_check_launch(ctx, "k_bs_markconv")) ||
+            (e = wd_launch_hop_f64_batch_c(ctx, xd, xd, Ud, 0, 1, HOP_M, 0.0, bd, nullptr,
+                                           nullptr, nullptr, chk_rec, n)) ||
+            (e = wd_launch_hop_f64_batch(ctx, t64d, xd, Ud, 0, -1, HOP_M, xd, nullptr, nullptr,
+                                         nullptr, chk, n)) ||
+            (e = wd_launch_hop_f64_batch(ctx, q64d, t64d, Ud, 1, -1, HOP_M, t64d, nullptr,
+                                         nullptr, nullptr, chk, n)))
+            return e;
+        hipLaunchKernelGGL(k_mx_fullres, dim3(eg_full, n), dim3(kBlock), 0, ctx->stream,
+                           (const double2 *)Mfull.f->data, (const double2 *)q64d, xe, stride,
+                           (const int *)check, pa);
+        hipLaunchKernelGGL(k_bs_fullfin, dim3(n), dim3(1024), 0, ctx->stream,
+                           (const double2 *)pa, eg_full, st, active, check, prm->tol,
+                           prm->max_iter);
+        hipLaunchKernelGGL(k_mx_alldone, dim3(1), dim3(64), 0, ctx->stream,
+                           (const int *)active, n, stop);
+        return wd_check_launch(ctx, "mixed bicgstab full check");
+    };
+
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    if (prm->use_graph) {
+        WD_HIP_CHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        const int e = block();
+        hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+        if (e) {
+            if (graph)
+                (void)hipGraphDestroy(graph);
+            return e;
+        }
+        WD_HIP_CHECK(ctx, ce);
+        ce = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+        if (ce != hipSuccess) {
+            (void)hipGraphDestroy(graph);
+            return wd_set_error(ctx, WD_EHIP, std::string("mixed bicgstab: graph instantiate: ") +
+                                                  hipGetErrorString(ce));
+        }
+    }
+    int status = WD_OK;
+    long launched = 0;
+    const long limit = (long)prm->max_iter + chunk;
+    int checks = 0;
+    bool checked = false;
+    while (true) {
+        WD_HIP_CHECK(ctx, hipMemcpyAsync(ctx->hpinned, stop, sizeof(int), hipMemcpyDeviceToHost,
+                                         ctx->stream));
+        WD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        const int hstop = *(int *)ctx->hpinned;
+        if (hstop && !checked && checks < 4) {
+            ++checks;
+            checked = true;
+            if ((status = full_check()))
+                break;
+            continue;
+        }
+        if (hstop || launched > limit)
+            break;
+        checked = false;
+        if (gexec) {
+            hipError_t ge = hipGraphLaunch(gexec, ctx->stream);
+            if (ge != hipSuccess) {
+                status = wd_set_error(ctx, WD_EHIP, std::string("mixed bicgstab: graph launch: ") +
+                                                        hipGetErrorString(ge));
+                break;
+            }
+        } else if ((status = block())) {
+            break;
+        }
+        launched += chunk;
+    }
+    if (gexec)
+        (void)hipGraphExecDestroy(gexec);
+    if (graph)
+        (void)hipGraphDestroy(graph);
+    if (status)
+        return status;
+    std::vector<BsState> hs(n);
+    WD_HIP_CHECK(ctx, hipMemcpy(hs.data(), st, sizeof(BsState) * n, hipMemcpyDeviceToHost));
+    const auto t1 = std::chrono::steady_clock::now();
+    if (seconds)
+        *seconds = std::chrono::duration<double>(t1 - t0).count();
+    bool all_conv = true, breakdown = false;
+    for (int k = 0; k < n; ++k) {
+        const double bn = hs[k].full_b, tr = hs[k].full_rsd;
+        const bool conv = hs[k].status == 1 && (!(bn > 0.0) || tr <= prm->tol * bn);
+        if (reports) {
+            reports[k].iterations = hs[k].iter;
+            reports[k].reliable_updates = hs[k].updates;
+            reports[k].converged = conv;
+            reports[k].true_rel_rsd = bn > 0 ? tr / bn : 0.0;
+            reports[k].rhs_norm = bn;
+            reports[k].rhs_per_thread = R;
+            reports[k].site_chunk = C;
+        }
+        all_conv = all_conv && conv;
+        breakdown = breakdown || hs[k].status == 2;
+    }
+    if (breakdown)
+        return wd_set_error(ctx, WD_EBREAKDOWN, "mixed bicgstab: breakdown in at least one system");
     return all_conv ? WD_OK : WD_ENOTCONV;
 }
 

@@ -48,7 +48,8 @@ EXPORTED = [
     "wd_norm2", "wd_dot",
     "wd_cg_params_default", "wd_cg", "wd_cg_eo", "wd_time_op", "wd_set_option",
     "wd_get_option",
-    "wd_mixed_params_default", "wd_cg_mixed", "wd_apply_M_sloppy", "wd_tune_report",
+    "wd_mixed_params_default", "wd_cg_mixed", "wd_bicgstab_mixed", "wd_apply_M_sloppy",
+    "wd_tune_report",
     "wd_pion_correlator",
     "wdg_gauge_random_su3", "wdg_gauge_near_identity", "wdg_gauge_unit",
     "wdg_gauge_flip_boundary", "wdg_spinor_gaussian", "wdg_spinor_point", "wdg_philox4x32",
@@ -155,6 +156,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "wd_mixed_params_default": ([ctypes.POINTER(MixedParams)], None),
         "wd_cg_mixed": ([vp, vp, vp, vp, vp, dp, ctypes.POINTER(MixedParams),
                          ctypes.POINTER(MixedReport), ctypes.POINTER(d)], i),
+        "wd_bicgstab_mixed": ([vp, vp, vp, vp, vp, dp, ctypes.POINTER(MixedParams),
+                               ctypes.POINTER(MixedReport), ctypes.POINTER(d)], i),
         "wd_apply_M_sloppy": ([vp, vp, vp, vp, dp, i, i, i], i),
         "wd_tune_report": ([vp, vp, vp, ctypes.POINTER(i), ctypes.POINTER(i), dp], i),
         "wd_time_op": ([vp, i, vp, vp, vp, d, i, i, i, ctypes.POINTER(d)], i),
@@ -351,6 +354,21 @@ class Context:
                  spinor_prec: int = F16, rhs_per_thread: int = 0, site_chunk: int = -1,
                  max_iter: int = 100000, check_every: int = 16, use_graph: bool = True,
                  even_odd: bool = False, reliable_every: int = 0):
+        return self._mixed(self.lib.wd_cg_mixed, "wd_cg_mixed", x, b, U, Us, masses, tol, delta,
+                           spinor_prec, rhs_per_thread, site_chunk, max_iter, check_every,
+                           use_graph, even_odd, reliable_every)
+
+    def bicgstab_mixed(self, x, b, U, Us, masses, tol: float = 1e-10, delta: float = 2e-2,
+                       spinor_prec: int = F16, rhs_per_thread: int = 0, site_chunk: int = -1,
+                       max_iter: int = 100000, check_every: int = 16, use_graph: bool = True,
+                       reliable_every: int = 0):
+        """wd_bicgstab_mixed: BiCGStab on the Schur system of every system."""
+        return self._mixed(self.lib.wd_bicgstab_mixed, "wd_bicgstab_mixed", x, b, U, Us, masses,
+                           tol, delta, spinor_prec, rhs_per_thread, site_chunk, max_iter,
+                           check_every, use_graph, True, reliable_every)
+
+    def _mixed(self, fn, name, x, b, U, Us, masses, tol, delta, spinor_prec, rhs_per_thread,
+               site_chunk, max_iter, check_every, use_graph, even_odd, reliable_every):
         p = MixedParams()
         self.lib.wd_mixed_params_default(ctypes.byref(p))
         p.max_iter, p.tol, p.delta = max_iter, tol, delta
@@ -361,9 +379,8 @@ class Context:
         assert m.size == b.nrhs
         reps = (MixedReport * b.nrhs)()
         sec = ctypes.c_double()
-        rc = self.lib.wd_cg_mixed(self.h, x.h, b.h, U.h, Us.h, _ptr(m), ctypes.byref(p), reps,
-                                  ctypes.byref(sec))
-        self.check(rc, "wd_cg_mixed")
+        rc = fn(self.h, x.h, b.h, U.h, Us.h, _ptr(m), ctypes.byref(p), reps, ctypes.byref(sec))
+        self.check(rc, name)
         return {
             "status": rc,
             "seconds": sec.value,

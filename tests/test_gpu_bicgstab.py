@@ -1,0 +1,107 @@
+"""Mixed-precision BiCGStab on the Schur system (wd_bicgstab_mixed).
+
+A new algorithm beside the reference's CGNR, so its parity target is the
+solution: every system must meet the reference's stop test on the full
+system, FP64 |M^dag b - M^dag M x| < 1e-10 |M^dag b|, re-checked here with the
+oracle's M (pinned to the compiled reference), and agree with wd_cg's
+solution.  Closed forms: zero links make A = a^2, so x = b / a; b = 0 gives
+x = 0.  Work orders (graph or direct launches, reliable-update slot spacing)
+must not change iterates.
+"""
+import numpy as np
+import pytest
+
+from oracle_py import kappa_to_mass, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _normal_residual(oracle, x, b, A, L, mass):
+    Mdb = oracle.dslash(b, A, L, mass, 1)
+    r = Mdb - oracle.dslash(oracle.dslash(x, A, L, mass, 0), A, L, mass, 1)
+    return np.linalg.norm(r) / np.linalg.norm(Mdb)
+
+
+@pytest.mark.parametrize("sprec,gprec,recon", [("F16", "F16", 12), ("F16", "F32", 12),
+                                                ("F32", "F32", 18)])
+@pytest.mark.parametrize("L", [[8, 8, 8, 8], [4, 4, 6, 8]])
+def test_bicgstab_reaches_fp64_true_residual(w, oracle, sprec, gprec, recon, L):
+    """Three systems at three kappas, antiperiodic t (a half volume of 384
+    sites included: not a multiple of the workgroup)."""
+    n = 3
+    A = w.gauge_near_identity(L, 61, 0.3)
+    bs = [w.spinor_gaussian(L, 62, r) for r in range(n)]
+    masses = [kappa_to_mass(k) for k in (0.12, 0.125, 0.13)]
+    print(f"bicgstab L={L} {sprec}/{gprec}/{recon}", flush=True)
+    with w.Context(L, bc=[1, 1, 1, -1]) as ctx:
+        U = ctx.gauge(A)
+        Us = ctx.gauge(A, prec=getattr(w, gprec), recon=recon)
+        fb = ctx.field(nrhs=n, data=bs)
+        X, Xe = ctx.field(nrhs=n), ctx.field(nrhs=n)
+        bi = ctx.bicgstab_mixed(X, fb, U, Us, masses, tol=1e-10, spinor_prec=getattr(w, sprec),
+                                check_every=4)
+        cg = ctx.cg_mixed(Xe, fb, U, Us, masses, tol=1e-10, spinor_prec=getattr(w, sprec),
+                          even_odd=True, check_every=4)
+        xs = [X.download(r) for r in range(n)]
+        ref = []
+        for r in range(n):
+            x1 = ctx.field()
+            ctx.cg(x1, ctx.field(data=bs[r]), U, masses[r], max_iter=5000, tol=1e-10,
+                   tol_relative=True, use_x0=False)
+            ref.append(x1.download())
+    assert bi["status"] == 0
+    Af = [a.copy() for a in A]
+    w.gauge_flip_boundary(L, 3, Af)
+    for r in range(n):
+        s, sc = bi["systems"][r], cg["systems"][r]
+        print(r, "bicgstab", s["iterations"], s["reliable_updates"], "cg", sc["iterations"],
+              s["true_rel_rsd"], flush=True)
+        assert s["converged"] and s["true_rel_rsd"] <= 1e-10
+        assert s["iterations"] < sc["iterations"]
+        assert _normal_residual(oracle, xs[r], bs[r], Af, L, masses[r]) < 1.05e-10
+        assert rel_l2(xs[r], ref[r]) < 1e-7
+
+
+def test_bicgstab_closed_forms(w):
+    """Zero links: A = a^2, the solution is b / a, found within a few
+    iterations (the first reliable update corrects the F32 accumulation of
+    x).  A zero right-hand side is solved by x = 0 without iterating."""
+    L = [4, 4, 4, 8]
+    zero = [np.zeros(w.volume(L) * 9, dtype=np.complex128) for _ in range(4)]
+    b = w.spinor_gaussian(L, 71)
+    mass = 0.3
+    with w.Context(L) as ctx:
+        U = ctx.gauge(zero)
+        Us = ctx.gauge(zero, prec=w.F32, recon=18)
+        fb = ctx.field(nrhs=2, data=[b, np.zeros_like(b)])
+        X = ctx.field(nrhs=2)
+        res = ctx.bicgstab_mixed(X, fb, U, Us, [mass, mass], spinor_prec=w.F32)
+        x0, x1 = X.download(0), X.download(1)
+    assert res["status"] == 0
+    assert res["systems"][0]["iterations"] <= 3
+    assert rel_l2(x0, b / (4 + mass)) < 1e-10
+    assert res["systems"][1]["iterations"] == 0 and np.all(x1 == 0)
+
+
+def test_bicgstab_work_orders_bitwise(w):
+    """Graph replay or direct launches, reliable-update slots after every
+    iteration or every 3rd: identical iterates and counts."""
+    L = [4, 4, 4, 8]
+    A = w.gauge_near_identity(L, 81, 0.3)
+    bs = [w.spinor_gaussian(L, 82, r) for r in range(2)]
+    masses = [kappa_to_mass(0.12), kappa_to_mass(0.13)]
+    out = []
+    with w.Context(L, bc=[1, 1, 1, -1]) as ctx:
+        U = ctx.gauge(A)
+        Us = ctx.gauge(A, prec=w.F16, recon=12)
+        fb = ctx.field(nrhs=2, data=bs)
+        for g, K in ((True, 1), (False, 1), (True, 3)):
+            X = ctx.field(nrhs=2)
+            res = ctx.bicgstab_mixed(X, fb, U, Us, masses, use_graph=g, check_every=3,
+                                     rhs_per_thread=2, reliable_every=K)
+            out.append(([X.download(r) for r in range(2)], res))
+            X.close()
+    for k in (1, 2):
+        for r in range(2):
+            assert np.array_equal(out[k][0][r], out[0][0][r])
+            assert out[k][1]["systems"][r]["iterations"] == out[0][1]["systems"][r]["iterations"]
