@@ -130,14 +130,16 @@ def pmc_traffic(L, tstream):
 
 
 # ---- GPU workloads -------------------------------------------------------------------
-CG_VARIANTS = ("fp64", "fp64_even_odd", "mixed", "mixed_even_odd")
+CG_VARIANTS = ("fp64", "fp64_even_odd", "mixed", "mixed_even_odd", "mixed_bicgstab")
 
 
 def cg_compare(w, ctx, U, A, b, mass, max_iter=20000, variants=CG_VARIANTS):
     """Time to solution of one system at 1e-10 relative residual (x0 = 0): the
     plain FP64 CG (the reference's algorithm, the baseline of the speedups),
-    FP64 CG on the Schur even-odd system, and the mixed-precision solver on the
-    full and on the Schur system.  Every solution is checked by the library
+    FP64 CG on the Schur even-odd system, the mixed-precision CG on the full
+    and on the Schur system, and mixed-precision BiCGStab on the Schur system
+    (a Krylov method beside CG, reported apart from the CG speedup).  Every
+    solution is checked by the library
     against the FP64 |M^dag b - M^dag M x| of the full system; the relative
     difference to the plain FP64 solution is reported beside it."""
     fb, x = ctx.field(data=b), ctx.field()
@@ -180,6 +182,22 @@ def cg_compare(w, ctx, U, A, b, mass, max_iter=20000, variants=CG_VARIANTS):
                     "rel_error_vs_fp64_solution": err(xm.download()),
                     "precision": f"FP32 arithmetic, F16 spinors, {label}, FP64 reliable updates"}
         del Us
+    if "mixed_bicgstab" in variants:
+        Us = ctx.gauge(A, prec=w.F16, recon=12)
+        for rep in range(2):  # the first call also autotunes the operator
+            xm = ctx.field()
+            mx = ctx.bicgstab_mixed(xm, fb, U, Us, [mass], tol=1e-10, spinor_prec=w.F16,
+                                    max_iter=max_iter)
+        s = mx["systems"][0]
+        out["mixed_bicgstab"] = {
+            "algorithm": "BiCGStab on the Schur complement (A itself, not A^dag A)",
+            "iterations": s["iterations"], "reliable_updates": s["reliable_updates"],
+            "seconds": round(mx["seconds"], 4), "solves_per_s": round(1.0 / mx["seconds"], 3),
+            "true_rel_residual": s["true_rel_rsd"],
+            "rel_error_vs_fp64_solution": err(xm.download()),
+            "precision": "FP32 arithmetic, F16 spinors, F16 12-real links, FP64 reliable updates"}
+        out["speedup_mixed_bicgstab_vs_fp64"] = round(fp["seconds"] / mx["seconds"], 3)
+        del Us
     mixed = [k for k in ("mixed", "mixed_even_odd") if k in out]
     if mixed:
         best = min(mixed, key=lambda k: out[k]["seconds"])
@@ -192,12 +210,13 @@ def cg_compare(w, ctx, U, A, b, mass, max_iter=20000, variants=CG_VARIANTS):
     return out
 
 
-def tune_solver(w, ctx, x, fb, U, Us, masses):
+def tune_solver(w, ctx, x, fb, U, Us, masses, even_odd=False):
     """Untimed warm-up: the first mixed solve of a context autotunes the sloppy
-    operator for this number of systems and link format (about 0.1 s); a solve
-    stopped after its first iteration block does that, so the timed solves
-    exclude it."""
-    ctx.cg_mixed(x, fb, U, Us, masses, tol=1e-10, spinor_prec=w.F16, max_iter=1)
+    operator for this number of systems, link format and operator (full
+    lattice, or one parity for the Schur system), about 0.1 s; a solve stopped
+    after its first iteration block does that, so the timed solves exclude it."""
+    ctx.cg_mixed(x, fb, U, Us, masses, tol=1e-10, spinor_prec=w.F16, max_iter=1,
+                 even_odd=even_odd)
 
 
 def batched_case(w):
@@ -225,8 +244,11 @@ def batched_case(w):
         log("batched: mixed solve of 48 systems on the Schur system")
         Us16 = ctx.gauge(A, prec=w.F16, recon=12)
         xe = ctx.field(nrhs=n)
-        tune_solver(w, ctx, xe, fb, U, Us16, masses)
+        tune_solver(w, ctx, xe, fb, U, Us16, masses, even_odd=True)
         mxe = ctx.cg_mixed(xe, fb, U, Us16, masses, tol=1e-10, spinor_prec=w.F16, even_odd=True)
+        log("batched: mixed BiCGStab of 48 systems on the Schur system")
+        xb = ctx.field(nrhs=n)
+        mxb = ctx.bicgstab_mixed(xb, fb, U, Us16, masses, tol=1e-10, spinor_prec=w.F16)
         del Us16
         sp = ctx.field(prec=w.F16, nrhs=n, data=srcs)
         bestR, bestC, ms = ctx.tune_report(sp, Us)
@@ -247,8 +269,11 @@ def batched_case(w):
         # effective mass, from the batched mixed Schur solve
         c_mx = np.array([ctx.pion_correlator(xe, site[3], 12 * k, 12)
                          for k in range(len(kappas))])
+        c_bi = np.array([ctx.pion_correlator(xb, site[3], 12 * k, 12)
+                         for k in range(len(kappas))])
     its = [s["iterations"] for s in mx["systems"]]
     its_e = [s["iterations"] for s in mxe["systems"]]
+    its_b = [s["iterations"] for s in mxb["systems"]]
     V = w.volume(L)
     return {
         "workload": "24x24x24x48, 12 point sources x kappa {0.120,0.125,0.130,0.135} = 48 systems",
@@ -260,6 +285,15 @@ def batched_case(w):
         "speedup_full_system": round(t_fp / mx["seconds"], 3),
         "all_converged": all(s["converged"] for s in mx["systems"] + mxe["systems"]),
         "max_true_rel_residual": max(s["true_rel_rsd"] for s in mx["systems"] + mxe["systems"]),
+        "mixed_bicgstab_seconds": round(mxb["seconds"], 3),
+        "mixed_bicgstab_solves_per_s": round(n / mxb["seconds"], 2),
+        "speedup_bicgstab": round(t_fp / mxb["seconds"], 3),
+        "bicgstab_all_converged": all(s["converged"] for s in mxb["systems"]),
+        "bicgstab_max_true_rel_residual": max(s["true_rel_rsd"] for s in mxb["systems"]),
+        "iterations_per_kappa_mixed_bicgstab": [int(np.max(its_b[12 * k:12 * k + 12]))
+                                                for k in range(4)],
+        "pion_correlator_max_diff_over_C0_bicgstab_vs_fp64":
+            float(np.max(np.abs(c_bi - c_fp) / c_fp[:, :1])),
         "iterations_per_kappa_mixed_even_odd": [int(np.max(its_e[12 * k:12 * k + 12]))
                                                 for k in range(4)],
         "iterations_per_kappa_mixed": [int(np.max(its[12 * k:12 * k + 12])) for k in range(4)],
@@ -299,7 +333,10 @@ def large_case(w):
         tune_solver(w, ctx, x, fb, U, Us, [mass] * n)
         mx = ctx.cg_mixed(x, fb, U, Us, [mass] * n, tol=1e-10, spinor_prec=w.F16)
         log("large: mixed solve of 12 systems on the Schur system")
+        tune_solver(w, ctx, x, fb, U, Us, [mass] * n, even_odd=True)
         mxe = ctx.cg_mixed(x, fb, U, Us, [mass] * n, tol=1e-10, spinor_prec=w.F16, even_odd=True)
+        log("large: mixed BiCGStab of 12 systems on the Schur system")
+        mxb = ctx.bicgstab_mixed(x, fb, U, Us, [mass] * n, tol=1e-10, spinor_prec=w.F16)
         # FP64 time to solution of one of the systems (all 12 share kappa and source statistics)
         b1 = ctx.field()
         b1.upload(w.spinor_gaussian(L, 20211026, 0))
@@ -323,6 +360,12 @@ def large_case(w):
         "speedup_full_system": round(fp["seconds"] * n / mx["seconds"], 3),
         "all_converged": all(s["converged"] for s in mx["systems"] + mxe["systems"]),
         "max_true_rel_residual": max(s["true_rel_rsd"] for s in mx["systems"] + mxe["systems"]),
+        "mixed_bicgstab_seconds": round(mxb["seconds"], 3),
+        "mixed_bicgstab_solves_per_s": round(n / mxb["seconds"], 3),
+        "speedup_bicgstab": round(fp["seconds"] * n / mxb["seconds"], 3),
+        "bicgstab_all_converged": all(s["converged"] for s in mxb["systems"]),
+        "bicgstab_max_true_rel_residual": max(s["true_rel_rsd"] for s in mxb["systems"]),
+        "iterations_mixed_bicgstab": max(s["iterations"] for s in mxb["systems"]),
         "iterations_mixed_even_odd": max(s["iterations"] for s in mxe["systems"]),
         "iterations_mixed": max(s["iterations"] for s in mx["systems"]),
         "iterations_fp64": fp["iterations"],

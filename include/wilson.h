@@ -65,6 +65,16 @@ const char *wd_last_error(const wd_ctx *ctx);
 int wd_sync(wd_ctx *ctx);
 /* Volume V = Lx*Ly*Lz*Lt of the context. */
 long wd_volume(const wd_ctx *ctx);
+/* Solver work memory (no counterpart in the reference, whose CGinvert
+ * constructs its work vectors on every call, src/invert.cpp:30-37).  The
+ * work vectors of a finished solve stay with the context and serve the next
+ * solve of the same shape: on the device a fresh allocation of many GB after
+ * earlier frees costs seconds.  When the device runs out of memory for a
+ * field, gauge or solve, the cached blocks are released and the allocation
+ * is retried.  wd_release_workspace frees them now; wd_workspace_bytes
+ * reports how many bytes are cached. */
+int wd_release_workspace(wd_ctx *ctx);
+int wd_workspace_bytes(const wd_ctx *ctx, size_t *bytes);
 
 /* ---- gauge field (lattice_gauge, include/lattice_gauge.h:11-26) ---------- */
 /* prec: WD_F64 | WD_F32 | WD_F16 ; recon: WD_RECON_18 | WD_RECON_12. */

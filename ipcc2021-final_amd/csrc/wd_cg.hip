@@ -201,6 +201,8 @@ __global__ __launch_bounds__(1024) void k_cg_init(const double2 *__restrict__ pa
     }
 }
 
+// a work vector of one solve, taken from and handed back to the context's
+// work cache
 struct Buf {
     wd_ctx *ctx = nullptr;
     double2 *p = nullptr;
@@ -208,17 +210,15 @@ struct Buf {
     ~Buf()
     {
         if (p)
-            (void)hipFree(p);
+            wd_work_put(ctx, p, sizeof(double2) * n);
     }
     int alloc(wd_ctx *c, size_t elems)
     {
         ctx = c;
         n = elems;
-        if (hipMalloc((void **)&p, sizeof(double2) * elems) != hipSuccess) {
-            (void)hipGetLastError();
-            p = nullptr;
+        p = (double2 *)wd_work_get(c, sizeof(double2) * elems);
+        if (!p)
             return wd_set_error(c, WD_ENOMEM, "cg: work vector allocation failed");
-        }
         return wd_launch_zero(c, p, sizeof(double2) * elems);
     }
 };

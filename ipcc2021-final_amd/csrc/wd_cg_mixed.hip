@@ -854,21 +854,25 @@ __global__ __launch_bounds__(1024) void k_bs_fullfin(const double2 *__restrict__
     }
 }
 
+// device scratch and work fields of one solve, taken from and handed back to
+// the context's work cache
 struct DevBuf {
+    wd_ctx *ctx = nullptr;
     void *p = nullptr;
+    size_t bytes = 0;
     ~DevBuf()
     {
         if (p)
-            (void)hipFree(p);
+            wd_work_put(ctx, p, bytes);
     }
-    int alloc(wd_ctx *c, size_t bytes)
+    int alloc(wd_ctx *c, size_t n)
     {
-        if (hipMalloc(&p, bytes) != hipSuccess) {
-            (void)hipGetLastError();
-            p = nullptr;
+        ctx = c;
+        bytes = (n + 15) / 16 * 16;
+        p = wd_work_get(c, bytes);
+        if (!p)
             return wd_set_error(c, WD_ENOMEM, "mixed cg: allocation failed");
-        }
-        return wd_launch_zero(c, p, (bytes + 15) / 16 * 16);
+        return wd_launch_zero(c, p, bytes);
     }
 };
 
@@ -1002,11 +1006,11 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
     // device work vectors (even-odd: the Schur system lives on the even half;
     // the odd half of an operator's output buffer holds its intermediate hop)
     FieldBuf Mdb, Mfull, t64, q64, rs, p, q, qq, xs;
-    if ((rc = wd_field_create(ctx, &Mdb.f, WD_F64, n)) || (rc = wd_field_create(ctx, &t64.f, WD_F64, n)) ||
-        (rc = wd_field_create(ctx, &q64.f, WD_F64, n)) || (rc = wd_field_create(ctx, &rs.f, sp, n)) ||
-        (rc = wd_field_create(ctx, &p.f, sp, n)) || (rc = wd_field_create(ctx, &q.f, sp, n)) ||
-        (rc = wd_field_create(ctx, &qq.f, sp, n)) || (rc = wd_field_create(ctx, &xs.f, WD_F32, n)) ||
-        (eo && (rc = wd_field_create(ctx, &Mfull.f, WD_F64, n))))
+    if ((rc = wd_field_create_work(ctx, &Mdb.f, WD_F64, n)) || (rc = wd_field_create_work(ctx, &t64.f, WD_F64, n)) ||
+        (rc = wd_field_create_work(ctx, &q64.f, WD_F64, n)) || (rc = wd_field_create_work(ctx, &rs.f, sp, n)) ||
+        (rc = wd_field_create_work(ctx, &p.f, sp, n)) || (rc = wd_field_create_work(ctx, &q.f, sp, n)) ||
+        (rc = wd_field_create_work(ctx, &qq.f, sp, n)) || (rc = wd_field_create_work(ctx, &xs.f, WD_F32, n)) ||
+        (eo && (rc = wd_field_create_work(ctx, &Mfull.f, WD_F64, n))))
         return rc;
     const int eg_full = (2 * stride + kBlock - 1) / kBlock; // elementwise workgroups per rhs
     // elementwise kernels cover (parity, site) e < nel: the even half only for
@@ -1412,12 +1416,12 @@ extern "C" int wd_bicgstab_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, co
     // F64: bh (even half), A x, M^dag b and the check's M^dag M x; sloppy: r, r0,
     // p, v, s, t (the odd halves of v and t hold the operator's intermediate hop)
     FieldBuf bh, Mfull, t64, q64, r, r0, pv, v, sv, t, xs;
-    if ((rc = wd_field_create(ctx, &bh.f, WD_F64, n)) || (rc = wd_field_create(ctx, &t64.f, WD_F64, n)) ||
-        (rc = wd_field_create(ctx, &q64.f, WD_F64, n)) || (rc = wd_field_create(ctx, &Mfull.f, WD_F64, n)) ||
-        (rc = wd_field_create(ctx, &r.f, sp, n)) || (rc = wd_field_create(ctx, &r0.f, sp, n)) ||
-        (rc = wd_field_create(ctx, &pv.f, sp, n)) || (rc = wd_field_create(ctx, &v.f, sp, n)) ||
-        (rc = wd_field_create(ctx, &sv.f, sp, n)) || (rc = wd_field_create(ctx, &t.f, sp, n)) ||
-        (rc = wd_field_create(ctx, &xs.f, WD_F32, n)))
+    if ((rc = wd_field_create_work(ctx, &bh.f, WD_F64, n)) || (rc = wd_field_create_work(ctx, &t64.f, WD_F64, n)) ||
+        (rc = wd_field_create_work(ctx, &q64.f, WD_F64, n)) || (rc = wd_field_create_work(ctx, &Mfull.f, WD_F64, n)) ||
+        (rc = wd_field_create_work(ctx, &r.f, sp, n)) || (rc = wd_field_create_work(ctx, &r0.f, sp, n)) ||
+        (rc = wd_field_create_work(ctx, &pv.f, sp, n)) || (rc = wd_field_create_work(ctx, &v.f, sp, n)) ||
+        (rc = wd_field_create_work(ctx, &sv.f, sp, n)) || (rc = wd_field_create_work(ctx, &t.f, sp, n)) ||
+        (rc = wd_field_create_work(ctx, &xs.f, WD_F32, n)))
         return rc;
     const int eg_full = (2 * stride + kBlock - 1) / kBlock;
     const int nel = stride; // the even half
@@ -1742,7 +1746,7 @@ extern "C" int wd_tune_report(wd_ctx *ctx, const wd_field *f, const wd_gauge *Us
     WD_ARG_CHECK(ctx, wd_sloppy_supported(f->prec, Us->prec, Us->recon),
                  "tune: unsupported precision combination");
     wd_field *out = nullptr;
-    int rc = wd_field_create(ctx, &out, f->prec, f->nrhs);
+    int rc = wd_field_create_work(ctx, &out, f->prec, f->nrhs);
     if (rc)
         return rc;
     DevBuf diag;

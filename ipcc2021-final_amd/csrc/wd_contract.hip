@@ -69,11 +69,9 @@ extern "C" int wd_pion_correlator(wd_ctx *ctx, const wd_field *S, int rhs_first,
     const int Lt = ctx->L[3];
     WD_ARG_CHECK(ctx, t0 >= 0 && t0 < Lt, "pion correlator: source time outside the lattice");
     const size_t n = (size_t)rhs_count * 2 * Lt;
-    double *d = nullptr;
-    if (hipMalloc((void **)&d, sizeof(double) * n) != hipSuccess) {
-        (void)hipGetLastError();
+    double *d = (double *)wd_work_get(ctx, sizeof(double) * n);
+    if (!d)
         return wd_set_error(ctx, WD_ENOMEM, "pion correlator: allocation failed");
-    }
     hipLaunchKernelGGL(k_slice_norm2, dim3(Lt, 2, rhs_count), dim3(kBlock), 0, ctx->stream,
                        (const double2 *)S->data, S->rhs_elems, ctx->geom, rhs_first, d);
     int rc = wd_check_launch(ctx, "k_slice_norm2");
@@ -86,7 +84,7 @@ extern "C" int wd_pion_correlator(wd_ctx *ctx, const wd_field *S, int rhs_first,
         if (e != hipSuccess)
             rc = wd_set_error(ctx, WD_EHIP, std::string("pion correlator: ") + hipGetErrorString(e));
     }
-    (void)hipFree(d);
+    wd_work_put(ctx, d, sizeof(double) * n);
     if (rc)
         return rc;
     for (int dt = 0; dt < Lt; ++dt) {

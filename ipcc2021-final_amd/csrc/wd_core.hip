@@ -63,9 +63,7 @@ int wd_ensure_staging(wd_ctx *ctx, size_t bytes)
         ctx->staging = nullptr;
         ctx->staging_bytes = 0;
     }
-    if (hipMalloc(&ctx->staging, bytes) != hipSuccess) {
-        ctx->staging = nullptr;
-        (void)hipGetLastError();
+    if (wd_malloc(ctx, &ctx->staging, bytes) != hipSuccess) {
         return wd_set_error(ctx, WD_ENOMEM, "staging buffer allocation failed");
     }
     ctx->staging_bytes = bytes;
@@ -150,6 +148,7 @@ extern "C" void wd_destroy(wd_ctx *c)
     (void)hipSetDevice(c->device);
     if (c->stream)
         (void)hipStreamSynchronize(c->stream);
+    wd_work_release(c);
     if (c->partials)
         (void)hipFree(c->partials);
     if (c->dscalars)
@@ -173,6 +172,70 @@ extern "C" int wd_sync(wd_ctx *ctx)
 
 extern "C" long wd_volume(const wd_ctx *ctx) { return ctx ? ctx->V : 0; }
 
+// ---- device memory ----------------------------------------------------------------
+hipError_t wd_malloc(wd_ctx *ctx, void **p, size_t bytes)
+{
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess && !ctx->work.empty()) {
+        (void)hipGetLastError();
+        wd_work_release(ctx);
+        e = hipMalloc(p, bytes);
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+    }
+    return e;
+}
+
+void *wd_work_get(wd_ctx *ctx, size_t bytes)
+{
+    for (size_t i = 0; i < ctx->work.size(); ++i)
+        if (ctx->work[i].first == bytes) {
+            void *p = ctx->work[i].second;
+            ctx->work[i] = ctx->work.back();
+            ctx->work.pop_back();
+            return p;
+        }
+    void *p = nullptr;
+    return wd_malloc(ctx, &p, bytes) == hipSuccess ? p : nullptr;
+}
+
+void wd_work_put(wd_ctx *ctx, void *p, size_t bytes)
+{
+    if (p)
+        ctx->work.emplace_back(bytes, p);
+}
+
+void wd_work_release(wd_ctx *ctx)
+{
+    if (ctx->work.empty())
+        return;
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto &w : ctx->work)
+        (void)hipFree(w.second);
+    ctx->work.clear();
+}
+
+extern "C" int wd_release_workspace(wd_ctx *ctx)
+{
+    if (!ctx)
+        return WD_EARG;
+    wd_work_release(ctx);
+    return WD_OK;
+}
+
+extern "C" int wd_workspace_bytes(const wd_ctx *ctx, size_t *bytes)
+{
+    if (!ctx || !bytes)
+        return WD_EARG;
+    size_t t = 0;
+    for (const auto &w : ctx->work)
+        t += w.first;
+    *bytes = t;
+    return WD_OK;
+}
+
 // ---- gauge --------------------------------------------------------------------------
 extern "C" int wd_gauge_create(wd_ctx *ctx, wd_gauge **U, int prec, int recon)
 {
@@ -191,8 +254,7 @@ extern "C" int wd_gauge_create(wd_ctx *ctx, wd_gauge **U, int prec, int recon)
     g->recon = recon;
     const int ncomp = recon == WD_RECON_18 ? 9 : 6;
     g->bytes = (size_t)4 * 2 * ncomp * ctx->stride * prec_bytes_complex(prec);
-    if (hipMalloc(&g->data, g->bytes) != hipSuccess) {
-        (void)hipGetLastError();
+    if (wd_malloc(ctx, &g->data, g->bytes) != hipSuccess) {
         delete g;
         return wd_set_error(ctx, WD_ENOMEM, "gauge allocation failed");
     }
@@ -235,7 +297,7 @@ extern "C" int wd_gauge_upload_ref(wd_ctx *ctx, wd_gauge *U, const double *const
 }
 
 // ---- fields -------------------------------------------------------------------------
-extern "C" int wd_field_create(wd_ctx *ctx, wd_field **f, int prec, int nrhs)
+static int field_create(wd_ctx *ctx, wd_field **f, int prec, int nrhs, bool work)
 {
     if (!ctx || !f)
         return WD_EARG;
@@ -248,21 +310,22 @@ extern "C" int wd_field_create(wd_ctx *ctx, wd_field **f, int prec, int nrhs)
     x->ctx = ctx;
     x->prec = prec;
     x->nrhs = nrhs;
+    x->work = work;
     x->rhs_elems = (size_t)24 * ctx->stride;
     x->bytes = x->rhs_elems * nrhs * prec_bytes_complex(prec);
-    if (hipMalloc(&x->data, x->bytes) != hipSuccess) {
-        (void)hipGetLastError();
-        delete x;
-        return wd_set_error(ctx, WD_ENOMEM, "field allocation failed");
+    const size_t sb = prec == WD_F16 ? sizeof(float) * 2 * ctx->stride * nrhs : 0;
+    bool ok;
+    if (work) {
+        x->data = wd_work_get(ctx, x->bytes);
+        x->scale = sb ? (float *)wd_work_get(ctx, sb) : nullptr;
+        ok = x->data && (!sb || x->scale);
+    } else {
+        ok = wd_malloc(ctx, &x->data, x->bytes) == hipSuccess &&
+             (!sb || wd_malloc(ctx, (void **)&x->scale, sb) == hipSuccess);
     }
-    if (prec == WD_F16) {
-        const size_t sb = sizeof(float) * 2 * ctx->stride * nrhs;
-        if (hipMalloc((void **)&x->scale, sb) != hipSuccess) {
-            (void)hipGetLastError();
-            (void)hipFree(x->data);
-            delete x;
-            return wd_set_error(ctx, WD_ENOMEM, "field scale allocation failed");
-        }
+    if (!ok) {
+        wd_field_destroy(x);
+        return wd_set_error(ctx, WD_ENOMEM, "field allocation failed");
     }
     int rc = wd_field_zero(ctx, x);
     if (rc) {
@@ -273,14 +336,32 @@ extern "C" int wd_field_create(wd_ctx *ctx, wd_field **f, int prec, int nrhs)
     return WD_OK;
 }
 
+extern "C" int wd_field_create(wd_ctx *ctx, wd_field **f, int prec, int nrhs)
+{
+    return field_create(ctx, f, prec, nrhs, false);
+}
+
+int wd_field_create_work(wd_ctx *ctx, wd_field **f, int prec, int nrhs)
+{
+    return field_create(ctx, f, prec, nrhs, true);
+}
+
 extern "C" void wd_field_destroy(wd_field *f)
 {
     if (!f)
         return;
-    (void)hipStreamSynchronize(f->ctx->stream);
-    (void)hipFree(f->data);
-    if (f->scale)
-        (void)hipFree(f->scale);
+    wd_ctx *ctx = f->ctx;
+    if (f->work) {
+        // stream order keeps the next user of the blocks behind this field's work
+        wd_work_put(ctx, f->data, f->bytes);
+        wd_work_put(ctx, f->scale, sizeof(float) * 2 * ctx->stride * f->nrhs);
+    } else {
+        (void)hipStreamSynchronize(ctx->stream);
+        if (f->data)
+            (void)hipFree(f->data);
+        if (f->scale)
+            (void)hipFree(f->scale);
+    }
     delete f;
 }
 

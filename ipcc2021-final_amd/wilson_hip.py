@@ -41,6 +41,7 @@ TUNE_NR, TUNE_NC = len(TUNE_R), len(TUNE_CHUNK)
 # Every symbol include/wilson.h and include/wilson_gen.h declare.
 EXPORTED = [
     "wd_create", "wd_destroy", "wd_strerror", "wd_last_error", "wd_sync", "wd_volume",
+    "wd_release_workspace", "wd_workspace_bytes",
     "wd_gauge_create", "wd_gauge_upload_ref", "wd_gauge_destroy",
     "wd_field_create", "wd_field_destroy", "wd_field_upload_ref", "wd_field_download_ref",
     "wd_field_zero", "wd_field_copy",
@@ -164,6 +165,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "wd_pion_correlator": ([vp, vp, i, i, i, dp], i),
         "wd_set_option": ([vp, i, i], i),
         "wd_get_option": ([vp, i, ctypes.POINTER(ctypes.c_int)], i),
+        "wd_release_workspace": ([vp], i),
+        "wd_workspace_bytes": ([vp, ctypes.POINTER(ctypes.c_size_t)], i),
         "wdg_gauge_random_su3": ([ip, ctypes.c_uint64, ctypes.POINTER(dp)], None),
         "wdg_gauge_near_identity": ([ip, ctypes.c_uint64, d, ctypes.POINTER(dp)], None),
         "wdg_gauge_unit": ([ip, ctypes.POINTER(dp)], None),
@@ -407,6 +410,15 @@ class Context:
     def get_option(self, option: int) -> int:
         v = ctypes.c_int()
         self.check(self.lib.wd_get_option(self.h, option, ctypes.byref(v)), "wd_get_option")
+        return v.value
+
+    def release_workspace(self):
+        """wd_release_workspace: free the solver work memory cached by the context."""
+        self.check(self.lib.wd_release_workspace(self.h), "wd_release_workspace")
+
+    def workspace_bytes(self) -> int:
+        v = ctypes.c_size_t()
+        self.check(self.lib.wd_workspace_bytes(self.h, ctypes.byref(v)), "wd_workspace_bytes")
         return v.value
 
     def norm2(self, f) -> float:
