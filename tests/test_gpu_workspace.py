@@ -60,3 +60,28 @@ def test_reused_work_memory_gives_identical_solves(w):
         # fields created after the solves start at zero
         f = ctx.field(nrhs=3)
         assert all(not np.any(f.download(k)) for k in range(3))
+
+
+def test_failed_allocation_releases_the_cache(w):
+    """A field larger than the device (about 630 GB: hipMalloc refuses it, no
+    kernel runs) fails with WD_ENOMEM after the cached work blocks were
+    released for the retry; solves afterwards still work."""
+    L = [8, 8, 8, 16]
+    A = w.gauge_near_identity(L, 73, 0.3)
+    b = w.spinor_gaussian(L, 74)
+    mass = kappa_to_mass(0.125)
+    print(f"workspace release on failed allocation L={L}", flush=True)
+    with w.Context(L, bc=[1, 1, 1, -1]) as ctx:
+        U = ctx.gauge(A)
+        fb = ctx.field(data=b)
+        x = ctx.field()
+        r1 = ctx.cg(x, fb, U, mass, max_iter=2000, tol=1e-10, tol_relative=True, use_x0=False)
+        x1 = x.download()
+        assert ctx.workspace_bytes() > 0
+        with pytest.raises(w.WilsonError) as e:
+            ctx.field(nrhs=400_000)
+        assert e.value.code == w.WD_ENOMEM
+        assert ctx.workspace_bytes() == 0
+        r2 = ctx.cg(x, fb, U, mass, max_iter=2000, tol=1e-10, tol_relative=True, use_x0=False)
+        assert r2["iterations"] == r1["iterations"]
+        assert np.array_equal(x.download(), x1)
