@@ -105,3 +105,36 @@ def test_bicgstab_work_orders_bitwise(w):
         for r in range(2):
             assert np.array_equal(out[k][0][r], out[0][0][r])
             assert out[k][1]["systems"][r]["iterations"] == out[0][1]["systems"][r]["iterations"]
+
+
+def test_bicgstab_iteration_limit_and_argument_errors(w):
+    """max_iter below what the systems need: status WD_ENOTCONV, no system
+    reported converged, at most max_iter iterations, a finite partial
+    solution.  Argument errors come back as WD_EARG before any launch."""
+    L = [4, 4, 4, 8]
+    A = w.gauge_near_identity(L, 91, 0.3)
+    bs = [w.spinor_gaussian(L, 92, r) for r in range(2)]
+    masses = [kappa_to_mass(0.13), kappa_to_mass(0.135)]
+    with w.Context(L, bc=[1, 1, 1, -1]) as ctx:
+        U = ctx.gauge(A)
+        Us = ctx.gauge(A, prec=w.F16, recon=12)
+        fb = ctx.field(nrhs=2, data=bs)
+        X = ctx.field(nrhs=2)
+        res = ctx.bicgstab_mixed(X, fb, U, Us, masses, max_iter=4, check_every=2)
+        assert res["status"] == w.WD_ENOTCONV
+        for r in range(2):
+            s = res["systems"][r]
+            assert not s["converged"] and 0 < s["iterations"] <= 4
+            assert np.all(np.isfinite(X.download(r)))
+        for kwargs, what in (({"tol": 0.0}, "tol"), ({"delta": 1.0}, "delta"),
+                             ({"rhs_per_thread": 3}, "rhs_per_thread"),
+                             ({"spinor_prec": w.F64}, "spinor_prec")):
+            with pytest.raises(w.WilsonError) as e:
+                ctx.bicgstab_mixed(X, fb, U, Us, masses, **kwargs)
+            assert e.value.code == w.WD_EARG, what
+        with pytest.raises(w.WilsonError) as e:
+            ctx.bicgstab_mixed(fb, fb, U, Us, masses)
+        assert e.value.code == w.WD_EARG
+        with pytest.raises(w.WilsonError) as e:   # sloppy links where U must be F64
+            ctx.bicgstab_mixed(X, fb, Us, Us, masses)
+        assert e.value.code == w.WD_EARG
