@@ -303,9 +303,14 @@ int wd_time_op(wd_ctx *ctx, int op, wd_field *out, const wd_field *in, const wd_
  * (Lt must be a multiple of T).  M and the hop are bit-identical to the
  * site-block kernel; the dot product's partial sums are grouped differently. */
 #define WD_OPT_TSTREAM 4
+/* FP64 hops over several systems (the mixed solvers' reliable updates and
+ * full-system checks): 1 (default) = two systems per thread, each link loaded
+ * once for both (links are three quarters of an FP64 hop's bytes); 0 = one
+ * system per thread.  Results are bit-identical. */
+#define WD_OPT_F64_PAIRS 5
 int wd_set_option(wd_ctx *ctx, int option, int value);
 /* The value an option has in effect on this context: 0/1 for
- * WD_OPT_XCD_REMAP and WD_OPT_PARITY_ORDER, the column width W (0 = natural
+ * WD_OPT_XCD_REMAP, WD_OPT_PARITY_ORDER and WD_OPT_F64_PAIRS, the column width W (0 = natural
  * order) for WD_OPT_BLOCK_TILE, the t-run length T (0 = site-block kernel)
  * for WD_OPT_TSTREAM. */
 int wd_get_option(wd_ctx *ctx, int option, int *value);

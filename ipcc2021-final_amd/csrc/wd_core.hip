@@ -123,6 +123,7 @@ extern "C" int wd_create(wd_ctx **out, const int L[4], const int bc[4], int devi
     c->xcd_remap = 1;
     c->parity_il = 1;
     c->ts_tb = wd_auto_tstream(c);
+    c->f64_pairs = 1;
     // partials: enough for a both-parity hop launch and the streaming
     // reduction grids (kStreamPartials, 256 KiB)
     const int hop_blocks = (int)(2 * ((Vh + kBlock - 1) / kBlock));
@@ -609,6 +610,9 @@ extern "C" int wd_get_option(wd_ctx *ctx, int option, int *value)
     case WD_OPT_TSTREAM:
         *value = ctx->ts_tb;
         return WD_OK;
+    case WD_OPT_F64_PAIRS:
+        *value = ctx->f64_pairs;
+        return WD_OK;
     default:
         return wd_set_error(ctx, WD_EARG, "get_option: unknown option");
     }
@@ -630,6 +634,9 @@ extern "C" int wd_set_option(wd_ctx *ctx, int option, int value)
         if (value >= 2)
             WD_ARG_CHECK(ctx, ctx->L[3] % value == 0, "set_option: Lt must be a multiple of the t-run length");
         ctx->ts_tb = value == 0 ? 0 : value == 1 ? wd_auto_tstream(ctx) : value;
+        return WD_OK;
+    case WD_OPT_F64_PAIRS:
+        ctx->f64_pairs = value ? 1 : 0;
         return WD_OK;
     case WD_OPT_BLOCK_TILE:
         WD_ARG_CHECK(ctx, value >= 0, "set_option: block tile width must be >= 0");

@@ -275,6 +275,172 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 1))) 
 }
 
 
+// One direction pair for two systems of a batch: each link is loaded once and
+// applied to both; per system the operations are those of hop_mu, in the
+// same order, so each system's result is bit-identical to k_hop_f64's.
+// use0 / use1 are uniform over the workgroup.
+template <int MU, bool DAG, bool NTL>
+__device__ __forceinline__ void hop_mu2(cpx<double> (&acc0)[4][3], cpx<double> (&acc1)[4][3],
+                                        bool use0, bool use1,
+                                        const double2 *__restrict__ in0,
+                                        const double2 *__restrict__ in1,
+                                        const double2 *__restrict__ U, const Geom &g, int p,
+                                        int i, int jf, int jb, double sf, double sb)
+{
+    const int stride = g.stride;
+    {
+        cpx<double> W[3][3];
+        load_link<NTL>(U + (long)((MU * 2 + p) * 9) * stride, stride, i, W);
+        using D = Dir<MU, true, DAG>;
+        if (use0) {
+            cpx<double> psi[4][3], h[2][3], r[2][3];
+            load_spinor(in0, stride, jf, psi);
+            project<D>(psi, h);
+            su3_mul_half<false>(W, h, r);
+            reconstruct<D>(acc0, r, sf);
+        }
+        if (use1) {
+            cpx<double> psi[4][3], h[2][3], r[2][3];
+            load_spinor(in1, stride, jf, psi);
+            project<D>(psi, h);
+            su3_mul_half<false>(W, h, r);
+            reconstruct<D>(acc1, r, sf);
+        }
+    }
+    pin_acc(acc0);
+    pin_acc(acc1);
+    {
+        cpx<double> W[3][3];
+        load_link<NTL>(U + (long)((MU * 2 + (1 - p)) * 9) * stride, stride, jb, W);
+        using D = Dir<MU, false, DAG>;
+        if (use0) {
+            cpx<double> psi[4][3], h[2][3], r[2][3];
+            load_spinor(in0, stride, jb, psi);
+            project<D>(psi, h);
+            su3_mul_half<true>(W, h, r);
+            reconstruct<D>(acc0, r, sb);
+        }
+        if (use1) {
+            cpx<double> psi[4][3], h[2][3], r[2][3];
+            load_spinor(in1, stride, jb, psi);
+            project<D>(psi, h);
+            su3_mul_half<true>(W, h, r);
+            reconstruct<D>(acc1, r, sb);
+        }
+    }
+    pin_acc(acc0);
+    pin_acc(acc1);
+}
+
+__device__ __forceinline__ void store_out(double2 *__restrict__ out_p,
+                                          const double2 *__restrict__ same_p, int i, double a,
+                                          double hc, const cpx<double> (&acc)[4][3], bool with_same)
+{
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            cpx<double> res;
+            if (with_same) {
+                const cpx<double> v = ld(same_p, f64_idx(s * 3 + c, (unsigned)i, 12));
+                res = {fma(a, v.r, hc * acc[s][c].r), fma(a, v.i, hc * acc[s][c].i)};
+            } else {
+                res = {hc * acc[s][c].r, hc * acc[s][c].i};
+            }
+            st(out_p, f64_idx(s * 3 + c, (unsigned)i, 12), res);
+        }
+}
+
+// Batched hop of two systems per thread (systems 2 blockIdx.y and
+// 2 blockIdx.y + 1): the link traffic, three quarters of an FP64 hop's bytes,
+// is paid once per pair.  HOP_ONLY and HOP_M; same work order, masks and
+// per-system coefficients as k_hop_f64 with a batch.
+template <bool DAG, int MODE, bool IL>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_hop_f64_pair(
+    double2 *__restrict__ out, const double2 *__restrict__ in, const double2 *__restrict__ U,
+    Geom g, int cb_first, int nparity, double a, double hc, const double2 *__restrict__ same,
+    int remap, const int *__restrict__ stop, F64Batch b, int nrhs)
+{
+    static_assert(MODE == HOP_ONLY || MODE == HOP_M, "pairs: HOP_ONLY and HOP_M only");
+    if (stop && *stop)
+        return;
+    const int r0 = 2 * blockIdx.y, r1 = r0 + 1;
+    const bool use0 = !b.mask || b.mask[r0];
+    const bool use1 = r1 < nrhs && (!b.mask || b.mask[r1]);
+    if (!use0 && !use1)
+        return;
+    const double a0 = b.a_rhs ? b.a_rhs[r0] : a;
+    const double a1 = (b.a_rhs && use1) ? b.a_rhs[r1] : a;
+    const double hc0 = b.c_rhs ? b.c_rhs[r0] : hc;
+    const double hc1 = (b.c_rhs && use1) ? b.c_rhs[r1] : hc;
+    const size_t ro0 = (size_t)r0 * b.rhs_elems;
+    const size_t ro1 = use1 ? (size_t)r1 * b.rhs_elems : ro0; // never read when !use1
+
+    const int lb = logical_block(remap);
+    const int bpp = gridDim.x / nparity;
+    int p, blk;
+    if (IL) {
+        p = lb & 1;
+        blk = block_order(lb >> 1, g);
+    } else {
+        p = (nparity == 2) ? lb / bpp : cb_first;
+        blk = block_order(lb - (nparity == 2 ? p * bpp : 0), g);
+    }
+    const int i = blk * kBlock + threadIdx.x;
+    if (i >= g.Vh)
+        return;
+    const int Lx = 2 * g.Lxh;
+    int rr = i / g.Lxh;
+    const int xh = i - rr * g.Lxh;
+    const int y = rr % g.Ly;
+    rr /= g.Ly;
+    const int z = rr % g.Lz;
+    const int t = rr / g.Lz;
+    const int x = 2 * xh + ((p + y + z + t) & 1);
+    const int row = i - xh;
+
+    const long po = (long)(1 - p) * 12 * g.stride;
+    const double2 *in0 = in + ro0 + po, *in1 = in + ro1 + po;
+    cpx<double> acc0[4][3], acc1[4][3];
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            acc0[s][c] = acc1[s][c] = {0.0, 0.0};
+    {
+        const int xf = (x + 1 == Lx) ? 0 : x + 1, xb = (x == 0) ? Lx - 1 : x - 1;
+        hop_mu2<0, DAG, !IL>(acc0, acc1, use0, use1, in0, in1, U, g, p, i, row + (xf >> 1),
+                             row + (xb >> 1), x == Lx - 1 ? g.bc[0] : 1.0, x == 0 ? g.bc[0] : 1.0);
+    }
+    {
+        const int s = g.Lxh;
+        const int jf = (y + 1 == g.Ly) ? i - (g.Ly - 1) * s : i + s;
+        const int jb = (y == 0) ? i + (g.Ly - 1) * s : i - s;
+        hop_mu2<1, DAG, !IL>(acc0, acc1, use0, use1, in0, in1, U, g, p, i, jf, jb,
+                             y == g.Ly - 1 ? g.bc[1] : 1.0, y == 0 ? g.bc[1] : 1.0);
+    }
+    {
+        const int s = g.Lxh * g.Ly;
+        const int jf = (z + 1 == g.Lz) ? i - (g.Lz - 1) * s : i + s;
+        const int jb = (z == 0) ? i + (g.Lz - 1) * s : i - s;
+        hop_mu2<2, DAG, !IL>(acc0, acc1, use0, use1, in0, in1, U, g, p, i, jf, jb,
+                             z == g.Lz - 1 ? g.bc[2] : 1.0, z == 0 ? g.bc[2] : 1.0);
+    }
+    {
+        const int s = g.Lxh * g.Ly * g.Lz;
+        const int jf = (t + 1 == g.Lt) ? i - (g.Lt - 1) * s : i + s;
+        const int jb = (t == 0) ? i + (g.Lt - 1) * s : i - s;
+        hop_mu2<3, DAG, !IL>(acc0, acc1, use0, use1, in0, in1, U, g, p, i, jf, jb,
+                             t == g.Lt - 1 ? g.bc[3] : 1.0, t == 0 ? g.bc[3] : 1.0);
+    }
+    const long oo = (long)p * 12 * g.stride;
+    constexpr bool WS = MODE == HOP_M;
+    if (use0)
+        store_out(out + ro0 + oo, WS ? same + ro0 + oo : nullptr, i, a0, hc0, acc0, WS);
+    if (use1)
+        store_out(out + ro1 + oo, WS ? same + ro1 + oo : nullptr, i, a1, hc1, acc1, WS);
+}
+
 // t-streaming kernel for the both-parity M / M^dagger launches of one system
 // (HOP_M, HOP_M_DOT).  A thread
 // owns the column (x, y, z) for Tb consecutive t.  Site (x, y, z, t) has
@@ -488,8 +654,27 @@ int launch_mode(wd_ctx *ctx, double2 *out, const double2 *in, const double2 *U, 
 #undef WD_HOP_TS_LAUNCH
         return wd_check_launch(ctx, "k_hop_f64_ts");
     }
-    const dim3 gd(grid, nrhs);
     const bool il = nparity == 2 && ctx->parity_il;
+    if (nrhs >= 2 && ctx->f64_pairs && (mode == HOP_ONLY || mode == HOP_M)) {
+        const dim3 gp(grid, (nrhs + 1) / 2);
+#define WD_HOP_PAIR_LAUNCH(M, I)                                                                   \
+    hipLaunchKernelGGL((k_hop_f64_pair<DAG, M, I>), gp, dim3(kBlock), 0, ctx->stream, out, in, U, \
+                       g, cb_first, nparity, a, hc, same, ctx->xcd_remap, stop, b, nrhs)
+        if (mode == HOP_ONLY) {
+            if (il)
+                WD_HOP_PAIR_LAUNCH(HOP_ONLY, true);
+            else
+                WD_HOP_PAIR_LAUNCH(HOP_ONLY, false);
+        } else {
+            if (il)
+                WD_HOP_PAIR_LAUNCH(HOP_M, true);
+            else
+                WD_HOP_PAIR_LAUNCH(HOP_M, false);
+        }
+#undef WD_HOP_PAIR_LAUNCH
+        return wd_check_launch(ctx, "k_hop_f64_pair");
+    }
+    const dim3 gd(grid, nrhs);
 #define WD_HOP_F64_LAUNCH(M, I)                                                                    \
     hipLaunchKernelGGL((k_hop_f64<DAG, M, I>), gd, dim3(kBlock), 0, ctx->stream, out, in, U, g,  \
                        cb_first, nparity, a, hc, same, dotv, partials, ctx->xcd_remap, stop, b)

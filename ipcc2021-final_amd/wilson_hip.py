@@ -32,6 +32,7 @@ OPT_XCD_REMAP = 1
 OPT_BLOCK_TILE = 2
 OPT_PARITY_ORDER = 3
 OPT_TSTREAM = 4
+OPT_F64_PAIRS = 5
 RECON_18, RECON_12 = 18, 12
 # autotuner candidates (include/wilson.h WD_TUNE_NR x WD_TUNE_NC)
 TUNE_R = (1, 2, 4)
