@@ -512,34 +512,6 @@ __global__ __launch_bounds__(kBlock) void k_bs_s(char *__restrict__ sv, float *_
     Spinor<SF>::template store<float, false>(sv + off, ss ? ss + so : nullptr, stride, (unsigned)i, vr);
 }
 
-// partial |t|^2 per (system, workgroup)
-template <int SF>
-__global__ __launch_bounds__(kBlock) void k_bs_norm(const char *__restrict__ t,
-                                                    const float *__restrict__ ts, size_t rhs_bytes,
-                                                    int stride, int nel,
-                                                    const int *__restrict__ active,
-                                                    double2 *__restrict__ partials)
-{
-    const int rhs = blockIdx.y;
-    if (active[rhs] != 1)
-        return;
-    double acc = 0.0;
-    if (blockIdx.x * kBlock + threadIdx.x < nel) {
-        BS_ELEM_PROLOGUE(SF)
-        cpx<float> vt[4][3];
-        Spinor<SF>::template load<float, false>(t + off, ts ? ts + so : nullptr, stride, (unsigned)i,
-                                               vt);
-#pragma unroll
-        for (int s = 0; s < 4; ++s)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                acc = fma((double)vt[s][c].r, (double)vt[s][c].r, acc);
-                acc = fma((double)vt[s][c].i, (double)vt[s][c].i, acc);
-            }
-    }
-    block_reduce_store(acc, 0.0, partials + (size_t)rhs * gridDim.x + blockIdx.x);
-}
-
 // x_s += alpha p + omega s; r = s - omega t; partials |r|^2 (pa) and (r0, r) (pb)
 // of the stored r
 template <int SF>
@@ -1531,7 +1503,8 @@ extern "C" int wd_bicgstab_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, co
             C = tC;
     }
     As.chunk = C;
-    auto apply_A = [&](wd_field *out, const wd_field *in, const wd_field *dotv) -> int {
+    // norm: also the partials of |out|^2 into pb
+    auto apply_A = [&](wd_field *out, const wd_field *in, const wd_field *dotv, bool norm) -> int {
         SloppyArgs m = As;
         m.out = (char *)out->data;
         m.out_scale = out->scale;
@@ -1546,7 +1519,9 @@ extern "C" int wd_bicgstab_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, co
             return e;
         m.in = (const char *)out->data; // a^2 in_e - D_eo (odd half of out) / 2
         m.in_scale = out->scale;
-        return wd_launch_hop_sloppy(ctx, m, sp, Us->prec, Us->recon, R, 0, 0, HOP_M_DOT);
+        m.partials2 = pb;
+        return wd_launch_hop_sloppy(ctx, m, sp, Us->prec, Us->recon, R, 0, 0,
+                                    norm ? HOP_M_DOT_NORM : HOP_M_DOT);
     };
     // FP64 A v on the even half (the odd half of out holds +D_oe v / 2)
     auto apply_A64 = [&](const double2 *in, double2 *out, const F64Batch &b1, const F64Batch &b2,
@@ -1572,19 +1547,17 @@ extern "C" int wd_bicgstab_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, co
         BS_SF(k_bs_p, (char *)pv.f->data, pv.f->scale, (const char *)r.f->data,
               (const float *)r.f->scale, (const char *)v.f->data, (const float *)v.f->scale, srb,
               stride, nel, (const BsState *)st, (const int *)active);
-        if ((e = apply_A(v.f, pv.f, r0.f)))
+        if ((e = apply_A(v.f, pv.f, r0.f, false)))
             return e;
         hipLaunchKernelGGL(k_bs_alpha, dim3(n), dim3(1024), 0, ctx->stream, (const double2 *)pa,
                            hop_nsb, st, active);
         BS_SF(k_bs_s, (char *)sv.f->data, sv.f->scale, (const char *)r.f->data,
               (const float *)r.f->scale, (const char *)v.f->data, (const float *)v.f->scale, srb,
               stride, nel, (const BsState *)st, (const int *)active);
-        if ((e = apply_A(t.f, sv.f, sv.f)))
+        if ((e = apply_A(t.f, sv.f, sv.f, true)))   // t = A s, (s, t) into pa, |t|^2 into pb
             return e;
-        BS_SF(k_bs_norm, (const char *)t.f->data, (const float *)t.f->scale, srb, stride, nel,
-              (const int *)active, pb);
         hipLaunchKernelGGL(k_bs_omega, dim3(n), dim3(1024), 0, ctx->stream, (const double2 *)pa,
-                           hop_nsb, (const double2 *)pb, eg, st, active);
+                           hop_nsb, (const double2 *)pb, hop_nsb, st, active);
         BS_SF(k_bs_update, (char *)xs.f->data, (char *)r.f->data, r.f->scale,
               (const char *)pv.f->data, (const float *)pv.f->scale, (const char *)sv.f->data,
               (const float *)sv.f->scale, (const char *)t.f->data, (const float *)t.f->scale,

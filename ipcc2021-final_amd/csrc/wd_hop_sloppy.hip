@@ -27,6 +27,8 @@
 //   HOP_M_DOT HOP_M plus per-(rhs, workgroup) FP64 partials of sum conj(dotv) out
 //             (per site in FP32 from the stored F16 values, whose products are
 //             exact in FP32; FP64 across sites)
+//   HOP_M_DOT_NORM  HOP_M_DOT plus the partials of |out|^2 (BiCGStab's |t|^2
+//             from the launch that writes t; no dagger form)
 #include "wd_fields.h"
 #include "wd_internal.h"
 #include "wd_sloppy.h"
@@ -400,7 +402,7 @@ __global__ __launch_bounds__(kBlock) void k_hop_sloppy(SloppyArgs a, Geom g, int
     for (int r = 0; r < R; ++r) {
         if (!act[r])
             continue;
-        double pr = 0.0, pi = 0.0;
+        double pr = 0.0, pi = 0.0, nn = 0.0;
         if (site_ok) {
             const size_t off = (size_t)rhs[r] * a.rhs_bytes + (size_t)p * pbytes;
             const size_t soff = (size_t)rhs[r] * 2 * stride + (size_t)p * stride;
@@ -431,7 +433,17 @@ __global__ __launch_bounds__(kBlock) void k_hop_sloppy(SloppyArgs a, Geom g, int
             if constexpr (SF == WD_F16) {
                 unsigned q[12];
                 const float sq = store16(a.out + off, a.out_scale + soff, stride, ui, acc[r], q);
-                if constexpr (MODE == HOP_M_DOT) {
+                if constexpr (MODE == HOP_M_DOT_NORM) {
+                    // |q|^2 of the stored values: f16 squares are exact in FP32
+                    float nq = 0.f;
+#pragma unroll
+                    for (int k = 0; k < 12; ++k) {
+                        nq = fmaf(re16(q[k]), re16(q[k]), nq);
+                        nq = fmaf(im16(q[k]), im16(q[k]), nq);
+                    }
+                    nn = (double)nq * ((double)sq * (double)sq);
+                }
+                if constexpr (MODE == HOP_M_DOT || MODE == HOP_M_DOT_NORM) {
                     // sum conj(d) q over the stored (rounded) values: f16 x f16
                     // products are exact in FP32, scales applied once
                     unsigned d[12];
@@ -451,7 +463,17 @@ __global__ __launch_bounds__(kBlock) void k_hop_sloppy(SloppyArgs a, Geom g, int
                 }
             } else {
                 store32(a.out + off, stride, ui, acc[r]);
-                if constexpr (MODE == HOP_M_DOT) {
+                if constexpr (MODE == HOP_M_DOT_NORM) {
+#pragma unroll
+                    for (int s = 0; s < 4; ++s)
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) {
+                            const double qr = acc[r][s][c].r, qi = acc[r][s][c].i;
+                            nn = fma(qr, qr, nn);
+                            nn = fma(qi, qi, nn);
+                        }
+                }
+                if constexpr (MODE == HOP_M_DOT || MODE == HOP_M_DOT_NORM) {
                     cpx<float> d[4][3];
                     load32(a.dotv + off, stride, ui, d);
 #pragma unroll
@@ -468,8 +490,10 @@ __global__ __launch_bounds__(kBlock) void k_hop_sloppy(SloppyArgs a, Geom g, int
                 }
             }
         }
-        if constexpr (MODE == HOP_M_DOT)
+        if constexpr (MODE == HOP_M_DOT || MODE == HOP_M_DOT_NORM)
             block_sum_store(pr, pi, a.partials + (size_t)rhs[r] * a.nsb + nat);
+        if constexpr (MODE == HOP_M_DOT_NORM)
+            block_sum_store(nn, 0.0, a.partials2 + (size_t)rhs[r] * a.nsb + nat);
     }
 }
 
@@ -485,6 +509,13 @@ void launch_r(wd_ctx *ctx, const SloppyArgs &a, const Geom &g, int cb_first, int
     case HOP_M:
         hipLaunchKernelGGL((k_hop_sloppy<SF, GF, R, DAG, HOP_M>), dim3(grid), dim3(kBlock), 0,
                            ctx->stream, a, g, cb_first, nparity, ctx->xcd_remap);
+        break;
+    case HOP_M_DOT_NORM:
+        // (s, t) and |t|^2 of BiCGStab's t = A s: no dagger form is needed
+        if constexpr (!DAG)
+            hipLaunchKernelGGL((k_hop_sloppy<SF, GF, R, false, HOP_M_DOT_NORM>), dim3(grid),
+                               dim3(kBlock), 0, ctx->stream, a, g, cb_first, nparity,
+                               ctx->xcd_remap);
         break;
     default:
         hipLaunchKernelGGL((k_hop_sloppy<SF, GF, R, DAG, HOP_M_DOT>), dim3(grid), dim3(kBlock), 0,
@@ -551,6 +582,8 @@ int wd_launch_hop_sloppy(wd_ctx *ctx, SloppyArgs a, int spinor_prec, int gauge_p
     // every per-parity array (at most 6 x 16 bytes per site) must fit 32-bit offsets
     WD_ARG_CHECK(ctx, 96.0 * g.stride < 4294967296.0, "sloppy hop: lattice too large");
     WD_ARG_CHECK(ctx, a.hc != 0.f, "sloppy hop: hop coefficient not set");
+    WD_ARG_CHECK(ctx, mode != HOP_M_DOT_NORM || (!dagger && a.partials2),
+                 "sloppy hop: the norm mode has no dagger form and needs partials2");
     a.one = 1.0f;
     WD_ARG_CHECK(ctx, wd_sloppy_supported(spinor_prec, gauge_prec, recon),
                  "sloppy hop: unsupported precision combination");

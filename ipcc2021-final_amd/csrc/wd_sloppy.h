@@ -18,6 +18,7 @@ struct SloppyArgs {
     const float *diag;       // [nrhs] a_r = 4 + m_r
     const int *active;       // [nrhs] 1 = apply to this rhs, else skip; nullptr = all
     double2 *partials;       // [nrhs][nsb] for HOP_M_DOT
+    double2 *partials2;      // [nrhs][nsb] (|out|^2, 0) for HOP_M_DOT_NORM
     const int *stop;         // whole launch becomes a no-op when *stop != 0
     int nrhs;
     float hc;                // hop coefficient: out = diag in + hc hop(in); -1/2 for M
