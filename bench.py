@@ -269,6 +269,7 @@ def batched_case(w):
         b1 = ctx.field(data=srcs[0])
         warm_fp64(ctx, ctx.field(), b1, U, masses[0])
         c_fp = np.zeros((len(kappas), L[3]))
+        err_e = err_b = 0.0
         for r in range(n):
             b1 = ctx.field(data=srcs[r])
             x1 = ctx.field()
@@ -277,6 +278,10 @@ def batched_case(w):
             t_fp += res["seconds"]
             its_fp.append(res["iterations"])
             c_fp[r // 12] += ctx.pion_correlator(x1, site[3])
+            x64 = x1.download()
+            nx = np.linalg.norm(x64)
+            err_e = max(err_e, float(np.linalg.norm(xe.download(r) - x64) / nx))
+            err_b = max(err_b, float(np.linalg.norm(xb.download(r) - x64) / nx))
         # the consumer of the 12 point sources per kappa: pion correlator and
         # effective mass, from the batched mixed Schur solve
         c_mx = np.array([ctx.pion_correlator(xe, site[3], 12 * k, 12)
@@ -302,6 +307,8 @@ def batched_case(w):
         "speedup_bicgstab": round(t_fp / mxb["seconds"], 3),
         "bicgstab_all_converged": all(s["converged"] for s in mxb["systems"]),
         "bicgstab_max_true_rel_residual": max(s["true_rel_rsd"] for s in mxb["systems"]),
+        "max_rel_error_vs_fp64_solution_mixed_even_odd": err_e,
+        "max_rel_error_vs_fp64_solution_bicgstab": err_b,
         "iterations_per_kappa_mixed_bicgstab": [int(np.max(its_b[12 * k:12 * k + 12]))
                                                 for k in range(4)],
         "pion_correlator_max_diff_over_C0_bicgstab_vs_fp64":
@@ -347,8 +354,10 @@ def large_case(w):
         log("large: mixed solve of 12 systems on the Schur system")
         tune_solver(w, ctx, x, fb, U, Us, [mass] * n, even_odd=True)
         mxe = ctx.cg_mixed(x, fb, U, Us, [mass] * n, tol=1e-10, spinor_prec=w.F16, even_odd=True)
+        xe0 = x.download(0)
         log("large: mixed BiCGStab of 12 systems on the Schur system")
         mxb = ctx.bicgstab_mixed(x, fb, U, Us, [mass] * n, tol=1e-10, spinor_prec=w.F16)
+        xb0 = x.download(0)
         # FP64 time to solution of one of the systems (all 12 share kappa and source statistics);
         # the mixed solvers' cached work memory (about 150 GB) is returned first
         ctx.release_workspace()
@@ -359,6 +368,11 @@ def large_case(w):
         log("large: FP64 CG of one system")
         fp = ctx.cg(x1, b1, U, mass, max_iter=20000, tol=1e-10, tol_relative=True, use_x0=False,
                     check_every=16)
+        x64 = x1.download()
+        nx = np.linalg.norm(x64)
+        err_e = float(np.linalg.norm(xe0 - x64) / nx)
+        err_b = float(np.linalg.norm(xb0 - x64) / nx)
+        del xe0, xb0, x64
         fin = ctx.field(prec=w.F16, nrhs=n)
         fin.copy_from(fb)
         bestR, bestC, ms = ctx.tune_report(fin, Us)
@@ -380,6 +394,8 @@ def large_case(w):
         "speedup_bicgstab": round(fp["seconds"] * n / mxb["seconds"], 3),
         "bicgstab_all_converged": all(s["converged"] for s in mxb["systems"]),
         "bicgstab_max_true_rel_residual": max(s["true_rel_rsd"] for s in mxb["systems"]),
+        "rel_error_vs_fp64_solution_system0_mixed_even_odd": err_e,
+        "rel_error_vs_fp64_solution_system0_bicgstab": err_b,
         "iterations_mixed_bicgstab": max(s["iterations"] for s in mxb["systems"]),
         "iterations_mixed_even_odd": max(s["iterations"] for s in mxe["systems"]),
         "iterations_mixed": max(s["iterations"] for s in mx["systems"]),
