@@ -138,3 +138,30 @@ def test_bicgstab_iteration_limit_and_argument_errors(w):
         with pytest.raises(w.WilsonError) as e:   # sloppy links where U must be F64
             ctx.bicgstab_mixed(X, fb, Us, Us, masses)
         assert e.value.code == w.WD_EARG
+
+
+def test_bicgstab_rhs_per_thread_and_chunk_bitwise(w):
+    """The autotuner's choices (systems per thread R, site-chunk order C) move
+    work between threads and workgroups only: five systems (groups of R = 4
+    leave one short), identical solutions and counts for every (R, C)."""
+    L = [4, 4, 8, 8]
+    n = 5
+    A = w.gauge_near_identity(L, 111, 0.3)
+    bs = [w.spinor_gaussian(L, 112, r) for r in range(n)]
+    masses = [kappa_to_mass(k) for k in (0.12, 0.122, 0.125, 0.128, 0.13)]
+    out = []
+    with w.Context(L, bc=[1, 1, 1, -1]) as ctx:
+        U = ctx.gauge(A)
+        Us = ctx.gauge(A, prec=w.F16, recon=12)
+        fb = ctx.field(nrhs=n, data=bs)
+        for R, C in ((1, 0), (2, 0), (4, 0), (2, 16), (4, 32)):
+            X = ctx.field(nrhs=n)
+            res = ctx.bicgstab_mixed(X, fb, U, Us, masses, rhs_per_thread=R, site_chunk=C)
+            assert res["status"] == 0 and res["systems"][0]["rhs_per_thread"] == R
+            out.append(([s["iterations"] for s in res["systems"]],
+                        [X.download(r) for r in range(n)]))
+            X.close()
+    for its, xs in out[1:]:
+        assert its == out[0][0]
+        for r in range(n):
+            assert np.array_equal(xs[r], out[0][1][r])
