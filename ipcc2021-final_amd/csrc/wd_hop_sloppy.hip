@@ -255,7 +255,9 @@ __device__ __forceinline__ float store16(char *base, float *scale, int stride, u
     const bool tiny = m < 1.0e-30f;
     const float inv = tiny ? 0.f : 1.0f / m;
     const float s = tiny ? 0.f : m;
-    *reinterpret_cast<float *>(reinterpret_cast<char *>(scale) + i * 4u) = s;
+    // outputs are streamed (non-temporal) so they do not evict from L2 the
+    // input spinors that neighbouring workgroups still read
+    __builtin_nontemporal_store(s, reinterpret_cast<float *>(reinterpret_cast<char *>(scale) + i * 4u));
 #pragma unroll
     for (int k = 0; k < 12; ++k)
         w[k] = f2h(v[k / 3][k % 3].r * inv, v[k / 3][k % 3].i * inv);
@@ -266,7 +268,7 @@ __device__ __forceinline__ float store16(char *base, float *scale, int stride, u
         c.y = w[4 * q + 1];
         c.z = w[4 * q + 2];
         c.w = w[4 * q + 3];
-        *reinterpret_cast<u4v *>(base + off16(q, stride, i)) = c;
+        __builtin_nontemporal_store(c, reinterpret_cast<u4v *>(base + off16(q, stride, i)));
     }
     return s;
 }
@@ -280,7 +282,7 @@ __device__ __forceinline__ void store32(char *base, int stride, unsigned i, cons
         c.y = v[(2 * q) / 3][(2 * q) % 3].i;
         c.z = v[(2 * q + 1) / 3][(2 * q + 1) % 3].r;
         c.w = v[(2 * q + 1) / 3][(2 * q + 1) % 3].i;
-        *reinterpret_cast<f4v *>(base + off16(q, stride, i)) = c;
+        __builtin_nontemporal_store(c, reinterpret_cast<f4v *>(base + off16(q, stride, i)));
     }
 }
 
