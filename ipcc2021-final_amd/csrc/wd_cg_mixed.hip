@@ -52,6 +52,10 @@ using namespace wdf;
 
 namespace {
 
+// The elementwise kernels stream whole fields with no reuse inside a launch:
+// non-temporal loads and stores keep them from churning L2.
+constexpr bool kStreamNT = true;
+
 struct MxState {
     double rho;      // |r|^2 carried by the recursion
     double alpha, beta;
@@ -90,14 +94,14 @@ __global__ __launch_bounds__(kBlock) void k_mx_xpay(char *__restrict__ p, float 
     const size_t so = ((size_t)rhs * 2 + par) * stride;
     const float beta = (float)st[rhs].beta;
     cpx<float> vp[4][3], vr[4][3];
-    Spinor<SF>::template load<float, false>(p + off, ps ? ps + so : nullptr, stride, (unsigned)i, vp);
-    Spinor<SF>::template load<float, false>(r + off, rs ? rs + so : nullptr, stride, (unsigned)i, vr);
+    Spinor<SF>::template load<float, kStreamNT>(p + off, ps ? ps + so : nullptr, stride, (unsigned)i, vp);
+    Spinor<SF>::template load<float, kStreamNT>(r + off, rs ? rs + so : nullptr, stride, (unsigned)i, vr);
 #pragma unroll
     for (int s = 0; s < 4; ++s)
 #pragma unroll
         for (int c = 0; c < 3; ++c)
             vp[s][c] = {fmaf(beta, vp[s][c].r, vr[s][c].r), fmaf(beta, vp[s][c].i, vr[s][c].i)};
-    Spinor<SF>::template store<float, false>(p + off, ps ? ps + so : nullptr, stride, (unsigned)i,
+    Spinor<SF>::template store<float, kStreamNT>(p + off, ps ? ps + so : nullptr, stride, (unsigned)i,
                                             vp);
 }
 
@@ -120,12 +124,12 @@ __global__ __launch_bounds__(kBlock) void k_mx_update(
         const size_t so = ((size_t)rhs * 2 + par) * stride;
         const float alpha = (float)st[rhs].alpha;
         cpx<float> vx[4][3], vr[4][3], vp[4][3], vq[4][3];
-        Spinor<WD_F32>::template load<float, false>(xs + xoff, nullptr, stride, (unsigned)i, vx);
-        Spinor<SF>::template load<float, false>(r + off, rs ? rs + so : nullptr, stride,
+        Spinor<WD_F32>::template load<float, kStreamNT>(xs + xoff, nullptr, stride, (unsigned)i, vx);
+        Spinor<SF>::template load<float, kStreamNT>(r + off, rs ? rs + so : nullptr, stride,
                                                (unsigned)i, vr);
-        Spinor<SF>::template load<float, false>(p + off, ps ? ps + so : nullptr, stride,
+        Spinor<SF>::template load<float, kStreamNT>(p + off, ps ? ps + so : nullptr, stride,
                                                (unsigned)i, vp);
-        Spinor<SF>::template load<float, false>(q + off, qs ? qs + so : nullptr, stride,
+        Spinor<SF>::template load<float, kStreamNT>(q + off, qs ? qs + so : nullptr, stride,
                                                (unsigned)i, vq);
 #pragma unroll
         for (int s = 0; s < 4; ++s)
@@ -136,15 +140,15 @@ __global__ __launch_bounds__(kBlock) void k_mx_update(
                 vr[s][c] = {fmaf(-alpha, vq[s][c].r, vr[s][c].r),
                             fmaf(-alpha, vq[s][c].i, vr[s][c].i)};
             }
-        Spinor<WD_F32>::template store<float, false>(xs + xoff, nullptr, stride, (unsigned)i, vx);
+        Spinor<WD_F32>::template store<float, kStreamNT>(xs + xoff, nullptr, stride, (unsigned)i, vx);
         // |r_s|^2 of the stored (rounded) residual
         if constexpr (SF == WD_F16) {
             unsigned w[12];
-            const float sc = Spinor<WD_F16>::template store_packed<float, false>(
+            const float sc = Spinor<WD_F16>::template store_packed<float, kStreamNT>(
                 r + off, rs + so, stride, (unsigned)i, vr, w);
             acc = norm2_packed(w, sc);
         } else {
-            Spinor<SF>::template store<float, false>(r + off, nullptr, stride, (unsigned)i, vr);
+            Spinor<SF>::template store<float, kStreamNT>(r + off, nullptr, stride, (unsigned)i, vr);
 #pragma unroll
             for (int s = 0; s < 4; ++s)
 #pragma unroll
@@ -471,9 +475,9 @@ __global__ __launch_bounds__(kBlock) void k_bs_p(char *__restrict__ p, float *__
     const float br = (float)st[rhs].beta.x, bi = (float)st[rhs].beta.y;
     const float wr = (float)st[rhs].omega.x, wi = (float)st[rhs].omega.y;
     cpx<float> vp[4][3], vr[4][3], vv[4][3];
-    Spinor<SF>::template load<float, false>(p + off, ps ? ps + so : nullptr, stride, (unsigned)i, vp);
-    Spinor<SF>::template load<float, false>(r + off, rs ? rs + so : nullptr, stride, (unsigned)i, vr);
-    Spinor<SF>::template load<float, false>(v + off, vs ? vs + so : nullptr, stride, (unsigned)i, vv);
+    Spinor<SF>::template load<float, kStreamNT>(p + off, ps ? ps + so : nullptr, stride, (unsigned)i, vp);
+    Spinor<SF>::template load<float, kStreamNT>(r + off, rs ? rs + so : nullptr, stride, (unsigned)i, vr);
+    Spinor<SF>::template load<float, kStreamNT>(v + off, vs ? vs + so : nullptr, stride, (unsigned)i, vv);
 #pragma unroll
     for (int s = 0; s < 4; ++s)
 #pragma unroll
@@ -483,7 +487,7 @@ __global__ __launch_bounds__(kBlock) void k_bs_p(char *__restrict__ p, float *__
             const float ui = vp[s][c].i - (wr * vv[s][c].i + wi * vv[s][c].r);
             vp[s][c] = {vr[s][c].r + (br * ur - bi * ui), vr[s][c].i + (br * ui + bi * ur)};
         }
-    Spinor<SF>::template store<float, false>(p + off, ps ? ps + so : nullptr, stride, (unsigned)i, vp);
+    Spinor<SF>::template store<float, kStreamNT>(p + off, ps ? ps + so : nullptr, stride, (unsigned)i, vp);
 }
 
 // s = r - alpha v
@@ -501,15 +505,15 @@ __global__ __launch_bounds__(kBlock) void k_bs_s(char *__restrict__ sv, float *_
     BS_ELEM_PROLOGUE(SF)
     const float ar = (float)st[rhs].alpha.x, ai = (float)st[rhs].alpha.y;
     cpx<float> vr[4][3], vv[4][3];
-    Spinor<SF>::template load<float, false>(r + off, rs ? rs + so : nullptr, stride, (unsigned)i, vr);
-    Spinor<SF>::template load<float, false>(v + off, vs ? vs + so : nullptr, stride, (unsigned)i, vv);
+    Spinor<SF>::template load<float, kStreamNT>(r + off, rs ? rs + so : nullptr, stride, (unsigned)i, vr);
+    Spinor<SF>::template load<float, kStreamNT>(v + off, vs ? vs + so : nullptr, stride, (unsigned)i, vv);
 #pragma unroll
     for (int s = 0; s < 4; ++s)
 #pragma unroll
         for (int c = 0; c < 3; ++c)
             vr[s][c] = {vr[s][c].r - (ar * vv[s][c].r - ai * vv[s][c].i),
                         vr[s][c].i - (ar * vv[s][c].i + ai * vv[s][c].r)};
-    Spinor<SF>::template store<float, false>(sv + off, ss ? ss + so : nullptr, stride, (unsigned)i, vr);
+    Spinor<SF>::template store<float, kStreamNT>(sv + off, ss ? ss + so : nullptr, stride, (unsigned)i, vr);
 }
 
 // x_s += alpha p + omega s; r = s - omega t; partials |r|^2 (pa) and (r0, r) (pb)
@@ -533,12 +537,12 @@ __global__ __launch_bounds__(kBlock) void k_bs_update(
         const float ar = (float)st[rhs].alpha.x, ai = (float)st[rhs].alpha.y;
         const float wr = (float)st[rhs].omega.x, wi = (float)st[rhs].omega.y;
         cpx<float> vx[4][3], vp[4][3], vs[4][3], vt[4][3];
-        Spinor<WD_F32>::template load<float, false>(xs + xoff, nullptr, stride, (unsigned)i, vx);
-        Spinor<SF>::template load<float, false>(p + off, ps ? ps + so : nullptr, stride, (unsigned)i,
+        Spinor<WD_F32>::template load<float, kStreamNT>(xs + xoff, nullptr, stride, (unsigned)i, vx);
+        Spinor<SF>::template load<float, kStreamNT>(p + off, ps ? ps + so : nullptr, stride, (unsigned)i,
                                                vp);
-        Spinor<SF>::template load<float, false>(sv + off, ss ? ss + so : nullptr, stride,
+        Spinor<SF>::template load<float, kStreamNT>(sv + off, ss ? ss + so : nullptr, stride,
                                                (unsigned)i, vs);
-        Spinor<SF>::template load<float, false>(t + off, ts ? ts + so : nullptr, stride, (unsigned)i,
+        Spinor<SF>::template load<float, kStreamNT>(t + off, ts ? ts + so : nullptr, stride, (unsigned)i,
                                                vt);
 #pragma unroll
         for (int s = 0; s < 4; ++s)
@@ -549,23 +553,23 @@ __global__ __launch_bounds__(kBlock) void k_bs_update(
                             vx[s][c].i + (ar * P.i + ai * P.r) + (wr * S.i + wi * S.r)};
                 vs[s][c] = {S.r - (wr * T.r - wi * T.i), S.i - (wr * T.i + wi * T.r)};
             }
-        Spinor<WD_F32>::template store<float, false>(xs + xoff, nullptr, stride, (unsigned)i, vx);
+        Spinor<WD_F32>::template store<float, kStreamNT>(xs + xoff, nullptr, stride, (unsigned)i, vx);
         cpx<float> vq[4][3]; // r as stored
         if constexpr (SF == WD_F16) {
             unsigned wv[12];
-            const float sc = Spinor<WD_F16>::template store_packed<float, false>(
+            const float sc = Spinor<WD_F16>::template store_packed<float, kStreamNT>(
                 r + off, rs + so, stride, (unsigned)i, vs, wv);
 #pragma unroll
             for (int k = 0; k < 12; ++k)
                 vq[k / 3][k % 3] = h2f(wv[k], sc);
         } else {
-            Spinor<SF>::template store<float, false>(r + off, nullptr, stride, (unsigned)i, vs);
+            Spinor<SF>::template store<float, kStreamNT>(r + off, nullptr, stride, (unsigned)i, vs);
 #pragma unroll
             for (int k = 0; k < 12; ++k)
                 vq[k / 3][k % 3] = vs[k / 3][k % 3];
         }
         cpx<float> v0[4][3];
-        Spinor<SF>::template load<float, false>(r0 + off, r0s ? r0s + so : nullptr, stride,
+        Spinor<SF>::template load<float, kStreamNT>(r0 + off, r0s ? r0s + so : nullptr, stride,
                                                (unsigned)i, v0);
 #pragma unroll
         for (int s = 0; s < 4; ++s)
@@ -620,9 +624,9 @@ __global__ __launch_bounds__(kBlock) void k_bs_relr(
         if (r0w)
             Spinor<SF>::store(r0 + off, r0s ? r0s + so : nullptr, stride, (unsigned)i, vb);
         cpx<float> vq[4][3], v0[4][3];
-        Spinor<SF>::template load<float, false>(r + off, rs ? rs + so : nullptr, stride, (unsigned)i,
+        Spinor<SF>::template load<float, kStreamNT>(r + off, rs ? rs + so : nullptr, stride, (unsigned)i,
                                                vq);
-        Spinor<SF>::template load<float, false>(r0 + off, r0s ? r0s + so : nullptr, stride,
+        Spinor<SF>::template load<float, kStreamNT>(r0 + off, r0s ? r0s + so : nullptr, stride,
                                                (unsigned)i, v0);
 #pragma unroll
         for (int s = 0; s < 4; ++s)
