@@ -440,9 +440,11 @@ extern "C" int wd_cg(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd_gauge
                                      hipMemcpyDeviceToHost, ctx->stream));
     WD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     const double true_rsd = std::sqrt(((double2 *)ctx->hpinned)->x);
-    if (params->rsd_history && hs.iter > 0)
-        WD_HIP_CHECK(ctx, hipMemcpy(params->rsd_history, d_hist, sizeof(double) * hs.iter,
-                                    hipMemcpyDeviceToHost));
+    if (params->rsd_history && hs.iter > 0) {
+        WD_HIP_CHECK(ctx, hipMemcpyAsync(params->rsd_history, d_hist, sizeof(double) * hs.iter,
+                                         hipMemcpyDeviceToHost, ctx->stream));
+        WD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    }
     const auto t1 = std::chrono::steady_clock::now();
     if (report) {
         report->iterations = hs.iter;
@@ -654,9 +656,11 @@ extern "C" int wd_cg_eo(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd_ga
             break;
         acc *= 0.5 * target / true_rsd;
     }
-    if (params->rsd_history && total > 0)
-        WD_HIP_CHECK(ctx, hipMemcpy(params->rsd_history, d_hist, sizeof(double) * total,
-                                    hipMemcpyDeviceToHost));
+    if (params->rsd_history && total > 0) {
+        WD_HIP_CHECK(ctx, hipMemcpyAsync(params->rsd_history, d_hist, sizeof(double) * total,
+                                         hipMemcpyDeviceToHost, ctx->stream));
+        WD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    }
     const auto t1 = std::chrono::steady_clock::now();
     const bool converged = true_rsd <= target && last_stop != 2;
     if (report) {

@@ -382,12 +382,13 @@ __global__ __launch_bounds__(1024) void k_mx_fullb(const double2 *__restrict__ p
 // Systems that met the Schur target (check[r]): accept when the full-system
 // residual meets tol, else resume them with a Schur target tightened by the
 // observed ratio (the state is that of a reliable update: r_s is the true
-// Schur residual, x_s = 0).
+// Schur residual, x_s = 0).  resume = 0 (the final pass) only records the
+// full-system residual.
 __global__ __launch_bounds__(1024) void k_mx_fullfin(const double2 *__restrict__ partials,
                                                      int count, MxState *__restrict__ st,
                                                      int *__restrict__ active,
                                                      int *__restrict__ check, double tol,
-                                                     int max_iter)
+                                                     int max_iter, int resume)
 {
     const int r = blockIdx.x;
     if (!check[r])
@@ -398,7 +399,7 @@ __global__ __launch_bounds__(1024) void k_mx_fullfin(const double2 *__restrict__
         const double full = sqrt(s.x);
         m.full_rsd = full;
         check[r] = 0;
-        if (full <= tol * m.full_b || !(m.full_b > 0.0) || m.iter >= max_iter ||
+        if (!resume || full <= tol * m.full_b || !(m.full_b > 0.0) || m.iter >= max_iter ||
             !isfinite(full))
             return;
         const double rr = m.true_rsd * m.true_rsd;
@@ -417,6 +418,24 @@ __global__ void k_mx_markconv(const MxState *__restrict__ st, int nrhs, int *__r
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r < nrhs)
         check[r] = st[r].status == 1 ? 1 : 0;
+}
+
+// Final pass of the Schur solvers: check[r] = 1 for every system whose x_o and
+// full-system residual do not yet belong to its final x_e -- systems stopped
+// at the iteration limit or by a breakdown (never checked), and systems that
+// a failed check resumed and that stopped after the last check.  Accepted
+// systems (converged and verified on the full system) keep their x_o.
+template <class State>
+__global__ void k_mark_final(const State *__restrict__ st, int nrhs, double tol,
+                             int *__restrict__ check)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nrhs) {
+        const State &m = st[r];
+        const bool accepted =
+            m.status == 1 && (!(m.full_b > 0.0) || m.full_rsd <= tol * m.full_b);
+        check[r] = accepted ? 0 : 1;
+    }
 }
 
 __global__ void k_mx_alldone(const int *__restrict__ active, int nrhs, int *__restrict__ stop)
@@ -808,7 +827,7 @@ __global__ __launch_bounds__(1024) void k_bs_fullfin(const double2 *__restrict__
                                                      int count, BsState *__restrict__ st,
                                                      int *__restrict__ active,
                                                      int *__restrict__ check, double tol,
-                                                     int max_iter)
+                                                     int max_iter, int resume)
 {
     const int r = blockIdx.x;
     if (!check[r])
@@ -819,7 +838,8 @@ __global__ __launch_bounds__(1024) void k_bs_fullfin(const double2 *__restrict__
         const double full = sqrt(s.x);
         m.full_rsd = full;
         check[r] = 0;
-        if (full <= tol * m.full_b || !(m.full_b > 0.0) || m.iter >= max_iter || !isfinite(full))
+        if (!resume || full <= tol * m.full_b || !(m.full_b > 0.0) || m.iter >= max_iter ||
+            !isfinite(full))
             return;
         m.target *= 0.5 * tol * m.full_b / full;
         m.rho = m.r0r;
@@ -856,6 +876,19 @@ struct FieldBuf {
     wd_field *f = nullptr;
     ~FieldBuf() { wd_field_destroy(f); }
 };
+
+// Reads the device stop flag after the work queued so far: returns 0 or 1,
+// or an error status (< 0) when the copy or the synchronisation failed.
+int poll_flag(wd_ctx *ctx, const int *stop, const char *who)
+{
+    hipError_t e = hipMemcpyAsync(ctx->hpinned, stop, sizeof(int), hipMemcpyDeviceToHost,
+                                  ctx->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess)
+        return wd_set_error(ctx, WD_EHIP, std::string(who) + ": poll: " + hipGetErrorString(e));
+    return *(const int *)ctx->hpinned ? 1 : 0;
+}
 
 } // namespace
 
@@ -1237,13 +1270,19 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
     };
 
     // even-odd: x_o = b_o / a + D_oe x_e / (2a) and |M^dag b - M^dag M x| for the
-    // systems that met the Schur target; the ones that miss tol resume
-    auto full_check = [&]() -> int {
+    // systems that met the Schur target; the ones that miss tol resume.  The
+    // final pass (final_pass = 1, after the loop) does the same for every
+    // system not yet accepted, without resuming any.
+    auto full_check = [&](int final_pass) -> int {
         int e;
         F64Batch chk_rec = {xe, check, inva_d, inv2a_d};
         F64Batch chk = {xe, check, ddiag, nullptr};
-        hipLaunchKernelGGL(k_mx_markconv, dim3((n + 255) / 256), dim3(256), 0, ctx->stream,
-                           (const MxState *)st, n, check);
+        if (final_pass)
+            hipLaunchKernelGGL(k_mark_final<MxState>, dim3((n + 255) / 256), dim3(256), 0,
+                               ctx->stream, (const MxState *)st, n, prm->tol, check);
+        else
+            hipLaunchKernelGGL(k_mx_markconv, dim3((n + 255) / 256), dim3(256), 0, ctx->stream,
+                               (const MxState *)st, n, check);
         if ((e = wd_check_launch(ctx, "k_mx_markconv")) ||
             (e = wd_launch_hop_f64_batch_c(ctx, xd, xd, Ud, 0, 1, HOP_M, 0.0, bd, nullptr,
                                            nullptr, nullptr, chk_rec, n)) ||
@@ -1257,9 +1296,10 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
                            (const int *)check, part);
         hipLaunchKernelGGL(k_mx_fullfin, dim3(n), dim3(1024), 0, ctx->stream,
                            (const double2 *)part, eg_full, st, active, check, prm->tol,
-                           prm->max_iter);
-        hipLaunchKernelGGL(k_mx_alldone, dim3(1), dim3(64), 0, ctx->stream,
-                           (const int *)active, n, stop);
+                           prm->max_iter, final_pass ? 0 : 1);
+        if (!final_pass)
+            hipLaunchKernelGGL(k_mx_alldone, dim3(1), dim3(64), 0, ctx->stream,
+                               (const int *)active, n, stop);
         return wd_check_launch(ctx, "mixed cg full check");
     };
 
@@ -1288,17 +1328,18 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
     int checks = 0;
     bool checked = false; // a full check ran since the last iterations
     while (true) {
-        int hstop = 0;
-        WD_HIP_CHECK(ctx, hipMemcpyAsync(ctx->hpinned, stop, sizeof(int), hipMemcpyDeviceToHost,
-                                         ctx->stream));
-        WD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-        hstop = *(int *)ctx->hpinned;
+        // a failed poll ends the loop like a failed launch, so the graph is
+        // always destroyed before the work blocks its nodes use are handed back
+        if ((status = poll_flag(ctx, stop, "mixed cg")) < 0)
+            break;
+        const int hstop = status;
+        status = WD_OK;
         if (hstop && eo && !checked && checks < 4) {
             // all systems stopped: verify the converged ones on the full system;
             // systems that miss tol are resumed and clear the stop flag
             ++checks;
             checked = true;
-            if ((status = full_check()))
+            if ((status = full_check(0)))
                 break;
             continue;
         }
@@ -1322,10 +1363,17 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
         (void)hipGraphExecDestroy(gexec);
     if (graph)
         (void)hipGraphDestroy(graph);
+    // even-odd: x_o and the full-system residual of every system not accepted
+    // by a check (stopped at the limit, broken down, or resumed after the last
+    // check), so each report holds the measured residual of the returned x
+    if (!status && eo)
+        status = full_check(1);
     if (status)
         return status;
     std::vector<MxState> hs(n);
-    WD_HIP_CHECK(ctx, hipMemcpy(hs.data(), st, sizeof(MxState) * n, hipMemcpyDeviceToHost));
+    WD_HIP_CHECK(ctx, hipMemcpyAsync(hs.data(), st, sizeof(MxState) * n, hipMemcpyDeviceToHost,
+                                     ctx->stream));
+    WD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     const auto t1 = std::chrono::steady_clock::now();
     if (seconds)
         *seconds = std::chrono::duration<double>(t1 - t0).count();
@@ -1605,13 +1653,18 @@ extern "C" int wd_bicgstab_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, co
     };
 
     // x_o = b_o / a + D_oe x_e / (2a) and |M^dag b - M^dag M x| for the systems
-    // that met the Schur target; the ones that miss tol resume
-    auto full_check = [&]() -> int {
+    // that met the Schur target; the ones that miss tol resume.  The final
+    // pass (after the loop) covers every system not yet accepted, resuming none.
+    auto full_check = [&](int final_pass) -> int {
         int e;
         F64Batch chk_rec = {xe, check, inva_d, inv2a_d};
         F64Batch chk = {xe, check, ddiag, nullptr};
-        hipLaunchKernelGGL(k_bs_markconv, dim3((n + 255) / 256), dim3(256), 0, ctx->stream,
-                           (const BsState *)st, n, check);
+        if (final_pass)
+            hipLaunchKernelGGL(k_mark_final<BsState>, dim3((n + 255) / 256), dim3(256), 0,
+                               ctx->stream, (const BsState *)st, n, prm->tol, check);
+        else
+            hipLaunchKernelGGL(k_bs_markconv, dim3((n + 255) / 256), dim3(256), 0, ctx->stream,
+                               (const BsState *)st, n, check);
         if ((e = wd_check_launch(ctx, "k_bs_markconv")) ||
             (e = wd_launch_hop_f64_batch_c(ctx, xd, xd, Ud, 0, 1, HOP_M, 0.0, bd, nullptr,
                                            nullptr, nullptr, chk_rec, n)) ||
@@ -1625,9 +1678,10 @@ extern "C" int wd_bicgstab_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, co
                            (const int *)check, pa);
         hipLaunchKernelGGL(k_bs_fullfin, dim3(n), dim3(1024), 0, ctx->stream,
                            (const double2 *)pa, eg_full, st, active, check, prm->tol,
-                           prm->max_iter);
-        hipLaunchKernelGGL(k_mx_alldone, dim3(1), dim3(64), 0, ctx->stream,
-                           (const int *)active, n, stop);
+                           prm->max_iter, final_pass ? 0 : 1);
+        if (!final_pass)
+            hipLaunchKernelGGL(k_mx_alldone, dim3(1), dim3(64), 0, ctx->stream,
+                               (const int *)active, n, stop);
         return wd_check_launch(ctx, "mixed bicgstab full check");
     };
 
@@ -1656,14 +1710,14 @@ extern "C" int wd_bicgstab_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, co
     int checks = 0;
     bool checked = false;
     while (true) {
-        WD_HIP_CHECK(ctx, hipMemcpyAsync(ctx->hpinned, stop, sizeof(int), hipMemcpyDeviceToHost,
-                                         ctx->stream));
-        WD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-        const int hstop = *(int *)ctx->hpinned;
+        if ((status = poll_flag(ctx, stop, "mixed bicgstab")) < 0)
+            break;
+        const int hstop = status;
+        status = WD_OK;
         if (hstop && !checked && checks < 4) {
             ++checks;
             checked = true;
-            if ((status = full_check()))
+            if ((status = full_check(0)))
                 break;
             continue;
         }
@@ -1686,10 +1740,15 @@ extern "C" int wd_bicgstab_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, co
         (void)hipGraphExecDestroy(gexec);
     if (graph)
         (void)hipGraphDestroy(graph);
+    // x_o and the full-system residual of every system not accepted by a check
+    if (!status)
+        status = full_check(1);
     if (status)
         return status;
     std::vector<BsState> hs(n);
-    WD_HIP_CHECK(ctx, hipMemcpy(hs.data(), st, sizeof(BsState) * n, hipMemcpyDeviceToHost));
+    WD_HIP_CHECK(ctx, hipMemcpyAsync(hs.data(), st, sizeof(BsState) * n, hipMemcpyDeviceToHost,
+                                     ctx->stream));
+    WD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     const auto t1 = std::chrono::steady_clock::now();
     if (seconds)
         *seconds = std::chrono::duration<double>(t1 - t0).count();
