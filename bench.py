@@ -587,6 +587,76 @@ def large_case(w, full_system=False):
     return out
 
 
+PROFILE_CASES = ("mixed32", "batched48", "batched48_bicgstab", "large12", "sloppy48")
+
+
+def profile_case(w, name):
+    """One solve (or a few operator launches) per process with a fixed work
+    order (no autotuning), for rocprofv3 kernel statistics and counters:
+      mixed32              32^3x64, one Gaussian system, mixed CG on M^dag M
+                           (F16 spinors, F32 12-real links), delta 1e-2
+      batched48            24^3x48, 48 point-source systems at 4 kappas, mixed
+                           CG on the Schur system (F16 spinors and links)
+      batched48_bicgstab   the same systems, mixed BiCGStab
+      large12              48^3x96, 12 Gaussian systems, mixed Schur CG
+      sloppy48             24^3x48, 48 systems: 10 launches of the sloppy M
+                           (F16 spinors, F16 12-real links, 2 systems per thread)
+    Prints one JSON line."""
+    t0 = time.perf_counter()
+    if name == "mixed32":
+        L, masses = [32, 32, 32, 64], [kappa_mass(0.135)]
+        srcs = [w.spinor_gaussian(L, 20211026)]
+    elif name in ("batched48", "batched48_bicgstab", "sloppy48"):
+        L = [24, 24, 24, 48]
+        rng = np.random.RandomState(20211027)
+        site = [int(rng.randint(0, n)) for n in L]
+        srcs = [w.spinor_point(L, site, s, c) for k in range(4) for s in range(4)
+                for c in range(3)]
+        masses = [kappa_mass(k) for k in (0.120, 0.125, 0.130, 0.135) for _ in range(12)]
+    else:
+        L, masses = [48, 48, 48, 96], [kappa_mass(0.135)] * 12
+        srcs = None
+    n = len(masses)
+    A = w.gauge_near_identity(L, 20211025, 0.3)
+    with w.Context(L, bc=[1, 1, 1, -1]) as ctx:
+        U = ctx.gauge(A)
+        Us = ctx.gauge(A, prec=w.F32 if name == "mixed32" else w.F16, recon=12)
+        if srcs is None:
+            fb = ctx.field(nrhs=n)
+            for r in range(n):
+                fb.upload(w.spinor_gaussian(L, 20211026, r), r)
+        else:
+            fb = ctx.field(nrhs=n, data=srcs)
+        del A
+        R = 1 if n == 1 else 2
+        if name == "sloppy48":
+            fin = ctx.field(prec=w.F16, nrhs=n)
+            fin.copy_from(fb)
+            fout = ctx.field(prec=w.F16, nrhs=n)
+            ctx.sync()
+            t1 = time.perf_counter()
+            for _ in range(10):
+                ctx.apply_M_sloppy(fout, fin, Us, masses, False, R, 0)
+            ctx.sync()
+            out = {"case": name, "launches": 10,
+                   "ms_per_launch": round((time.perf_counter() - t1) * 100, 4)}
+        else:
+            x = ctx.field(nrhs=n)
+            if name == "batched48_bicgstab":
+                res = ctx.bicgstab_mixed(x, fb, U, Us, masses, tol=1e-10, delta=DELTA,
+                                         rhs_per_thread=R, site_chunk=0)
+            else:
+                res = ctx.cg_mixed(x, fb, U, Us, masses, tol=1e-10, delta=DELTA,
+                                   rhs_per_thread=R, site_chunk=0, even_odd=name != "mixed32")
+            require_converged(res, name)
+            out = {"case": name, "systems": n, "seconds": round(res["seconds"], 4),
+                   "iterations": max(s["iterations"] for s in res["systems"]),
+                   "max_true_rel_residual": max(s["true_rel_rsd"] for s in res["systems"])}
+    out["lattice"] = L
+    out["wall_s"] = round(time.perf_counter() - t0, 2)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -607,10 +677,15 @@ def main():
     ap.add_argument("--cg", nargs="+", choices=CG_VARIANTS, default=list(CG_VARIANTS),
                     help="CG solvers timed (the plain FP64 CG always runs: it is the baseline)")
     ap.add_argument("--cpu-worker", default=None)
+    ap.add_argument("--profile-case", choices=PROFILE_CASES, default=None,
+                    help="run one solve or operator case for rocprofv3 and exit")
     args = ap.parse_args()
 
     if args.cpu_worker:
         cpu_baseline_worker(args.lattice, args.cpu_worker, args.cpu_seconds)
+        return
+    if args.profile_case:
+        profile_case(load_pkg().wilson_hip, args.profile_case)
         return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; multi-GPU runs (--gpus > 1) are not "
