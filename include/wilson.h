@@ -268,6 +268,18 @@ int wd_tune_report(wd_ctx *ctx, const wd_field *f, const wd_gauge *Us, int *best
 int wd_pion_correlator(wd_ctx *ctx, const wd_field *S, int rhs_first, int rhs_count, int t0,
                        double *Ct);
 
+/* wd_pion_correlator without any allocation: no counterpart in the reference.
+ * For callers that cannot allocate device or host memory (tight serving
+ * loops after each batched solve, code that forbids hipMalloc).  Same
+ * arguments and the same result, bit for bit; the per-slice partial sums live
+ * in a fixed device array of the library (16384 + 512 doubles, one per GPU,
+ * held until the process ends), so rhs_count * 2 * Lt must not exceed 16384
+ * and Lt not 512 (WD_EARG otherwise).  Every call on one GPU shares that
+ * array: calls that overlap on one GPU (contexts used from several host
+ * threads) race and must be serialised by the caller. */
+int wd_pion_correlator_noalloc(wd_ctx *ctx, const wd_field *S, int rhs_first, int rhs_count,
+                               int t0, double *Ct);
+
 /* ---- measurement and tuning --------------------------------------------------
  * No counterpart in the reference, which times whole phases with MPI_Wtime
  * (src/main.cpp:88-110). */

@@ -138,6 +138,9 @@ extern "C" int wd_create(wd_ctx **out, const int L[4], const int bc[4], int devi
         wd_destroy(c);
         return WD_ENOMEM;
     }
+    // the allocation-free correlator's scratch address on this device; a
+    // failure here only disables that entry point (it then returns an error)
+    (void)wd_contract_setup(device);
     *out = c;
     return WD_OK;
 }

@@ -172,6 +172,9 @@ int wd_launch_hop_f64(wd_ctx *ctx, double2 *out, const double2 *in, const double
                       double2 *partials, const int *stop);
 int wd_launch_zero(wd_ctx *ctx, void *p, size_t bytes);
 int wd_auto_tstream(const wd_ctx *ctx);
+// device address of the allocation-free correlator's scratch, once per device
+// (wd_contract.hip); called by wd_create with the device current
+int wd_contract_setup(int device);
 int wd_ensure_staging(wd_ctx *ctx, size_t bytes);
 
 // layout / precision conversion (wd_convert.hip)

@@ -52,7 +52,7 @@ EXPORTED = [
     "wd_get_option",
     "wd_mixed_params_default", "wd_cg_mixed", "wd_bicgstab_mixed", "wd_apply_M_sloppy",
     "wd_tune_report",
-    "wd_pion_correlator",
+    "wd_pion_correlator", "wd_pion_correlator_noalloc",
     "wdg_gauge_random_su3", "wdg_gauge_near_identity", "wdg_gauge_unit",
     "wdg_gauge_flip_boundary", "wdg_spinor_gaussian", "wdg_spinor_point", "wdg_philox4x32",
 ]
@@ -164,6 +164,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "wd_tune_report": ([vp, vp, vp, ctypes.POINTER(i), ctypes.POINTER(i), dp], i),
         "wd_time_op": ([vp, i, vp, vp, vp, d, i, i, i, ctypes.POINTER(d)], i),
         "wd_pion_correlator": ([vp, vp, i, i, i, dp], i),
+        "wd_pion_correlator_noalloc": ([vp, vp, i, i, i, dp], i),
         "wd_set_option": ([vp, i, i], i),
         "wd_get_option": ([vp, i, ctypes.POINTER(ctypes.c_int)], i),
         "wd_release_workspace": ([vp], i),
@@ -403,6 +404,17 @@ class Context:
         Ct = np.zeros(self.L[3])
         self.check(self.lib.wd_pion_correlator(self.h, S.h, rhs_first, rhs_count, t0, _ptr(Ct)),
                    "wd_pion_correlator")
+        return Ct
+
+    def pion_correlator_noalloc(self, S, t0: int, rhs_first: int = 0,
+                                rhs_count: Optional[int] = None):
+        """As pion_correlator, through wd_pion_correlator_noalloc (no allocation;
+        rhs_count * 2 * Lt <= 16384, Lt <= 512)."""
+        if rhs_count is None:
+            rhs_count = S.nrhs - rhs_first
+        Ct = np.zeros(self.L[3])
+        self.check(self.lib.wd_pion_correlator_noalloc(self.h, S.h, rhs_first, rhs_count, t0,
+                                                       _ptr(Ct)), "wd_pion_correlator_noalloc")
         return Ct
 
     def set_option(self, option: int, value: int):
