@@ -311,6 +311,39 @@ def tune_solver(w, ctx, x, fb, U, Us, masses, even_odd=False):
                  even_odd=even_odd)
 
 
+def loop_kernel_case(w, ctx, fb, U, A, fin, masses):
+    """The sloppy operator with every system of a site in one thread (links
+    loaded once per site, packed-half products) against the tuned R systems
+    per thread, both on F16 spinors and F16 12-real links: the full M and the
+    one-parity launch of the Schur solvers, and the batched Schur CG and
+    BiCGStab solves on each (every system to 1e-10 on the full system)."""
+    n = fin.nrhs
+    V = w.volume(fin.ctx.L)
+    Us = ctx.gauge(A, prec=w.F16, recon=12)
+    fout = ctx.field(prec=w.F16, nrhs=n)
+    Rt = 1 if n == 1 else 2
+    out = {"links": "F16 12-real", "tuned_rhs_per_thread": Rt}
+    for key, R in (("tuned", Rt), ("loop", w.RHS_LOOP)):
+        full = ctx.time_sloppy(fout, fin, Us, masses, R, 0, -1, 10)
+        half = ctx.time_sloppy(fout, fin, Us, masses, R, 0, 0, 10)
+        out[f"M_ms_{key}"] = round(full, 4)
+        out[f"one_parity_ms_{key}"] = round(half, 4)
+        # links once per site, input spinor twice (neighbours + diagonal), output once
+        out[f"M_GBps_{key}"] = round((8 * 24 / n + 3 * 52) * V * n / (full * 1e-3) / 1e9, 1)
+        x = ctx.field(nrhs=n)
+        cg = require_converged(ctx.cg_mixed(x, fb, U, Us, masses, tol=1e-10, rhs_per_thread=R,
+                                            site_chunk=0, even_odd=True), f"Schur CG R={R}")
+        bi = require_converged(ctx.bicgstab_mixed(x, fb, U, Us, masses, tol=1e-10,
+                                                  rhs_per_thread=R, site_chunk=0),
+                               f"BiCGStab R={R}")
+        out[f"mixed_even_odd_seconds_{key}"] = round(cg["seconds"], 4)
+        out[f"mixed_bicgstab_seconds_{key}"] = round(bi["seconds"], 4)
+        out[f"max_iterations_{key}"] = [max(s["iterations"] for s in cg["systems"]),
+                                        max(s["iterations"] for s in bi["systems"])]
+    out["M_frac_of_8TBps_loop"] = round(out["M_GBps_loop"] / HBM_PEAK_GBS, 4)
+    return out
+
+
 def batched32_case(w, L=(32, 32, 32, 64), n=12, kappa=0.135):
     """n Gaussian systems at one kappa on 32^3x64 in one batched mixed solve
     (FP32 arithmetic, F16 spinors, FP64 reliable updates at every residual drop
@@ -430,6 +463,7 @@ def batched_case(w):
         del Us16
         sp = ctx.field(prec=w.F16, nrhs=n, data=srcs)
         bestR, bestC, ms = ctx.tune_report(sp, Us)
+        loop = loop_kernel_case(w, ctx, fb, U, A, sp, masses)
         # FP64 baseline: the reference algorithm, one system after the other
         log("batched: FP64 CG of the 48 systems one by one")
         t_fp = 0.0
@@ -498,6 +532,7 @@ def batched_case(w):
         "sloppy_M_ms_by_R_chunk": {f"{R}/{C}": round(v, 4) for (R, C), v in ms.items()},
         "sloppy_M_GBps_best": round((8 * 48 / n + 3 * 52) * V * n / (ms[(bestR, bestC)] * 1e-3)
                                     / 1e9, 1),
+        "loop_kernel": loop,
     }
 
 
@@ -551,6 +586,9 @@ def large_case(w, full_system=False):
         fin = ctx.field(prec=w.F16, nrhs=n)
         fin.copy_from(fb)
         bestR, bestC, ms = ctx.tune_report(fin, Us)
+        fout = ctx.field(prec=w.F16, nrhs=n)
+        ms_loop = ctx.time_sloppy(fout, fin, Us, [mass] * n, w.RHS_LOOP, 0, -1, 5)
+        del fout
     V = w.volume(L)
     bytes_site = 8 * 24 / n + 3 * 52   # F16 12-real links shared by n systems, F16 spinors
     cg_systems = mxe["systems"] + (mx["systems"] if mx else [])
@@ -578,6 +616,8 @@ def large_case(w, full_system=False):
         "site_chunk": mxe["systems"][0]["site_chunk"],
         "sloppy_M_ms_by_R_chunk": {f"{R}/{C}": round(v, 3) for (R, C), v in ms.items()},
         "sloppy_M_GBps_best": round(bytes_site * V * n / (ms[(bestR, bestC)] * 1e-3) / 1e9, 1),
+        "sloppy_M_ms_loop_kernel": round(ms_loop, 3),
+        "sloppy_M_GBps_loop_kernel": round(bytes_site * V * n / (ms_loop * 1e-3) / 1e9, 1),
     }
     if mx:
         out.update({"mixed_seconds": round(mx["seconds"], 3),
@@ -587,23 +627,27 @@ def large_case(w, full_system=False):
     return out
 
 
-PROFILE_CASES = ("mixed32", "batched48", "batched48_bicgstab", "large12", "sloppy48")
+PROFILE_CASES = ("mixed32", "eo32", "bicgstab32", "batched48", "batched48_bicgstab", "large12",
+                 "sloppy48")
 
 
-def profile_case(w, name):
+def profile_case(w, name, R=None):
     """One solve (or a few operator launches) per process with a fixed work
     order (no autotuning), for rocprofv3 kernel statistics and counters:
       mixed32              32^3x64, one Gaussian system, mixed CG on M^dag M
                            (F16 spinors, F32 12-real links), delta 1e-2
+      eo32 / bicgstab32    the same system, mixed CG on the Schur system / mixed
+                           BiCGStab (F16 spinors and F16 12-real links)
       batched48            24^3x48, 48 point-source systems at 4 kappas, mixed
                            CG on the Schur system (F16 spinors and links)
       batched48_bicgstab   the same systems, mixed BiCGStab
       large12              48^3x96, 12 Gaussian systems, mixed Schur CG
       sloppy48             24^3x48, 48 systems: 10 launches of the sloppy M
                            (F16 spinors, F16 12-real links, 2 systems per thread)
-    Prints one JSON line."""
+    R: systems per thread of the sloppy operator (1, 2, 4 or w.RHS_LOOP);
+    default 1 for one system, 2 for many.  Prints one JSON line."""
     t0 = time.perf_counter()
-    if name == "mixed32":
+    if name in ("mixed32", "eo32", "bicgstab32"):
         L, masses = [32, 32, 32, 64], [kappa_mass(0.135)]
         srcs = [w.spinor_gaussian(L, 20211026)]
     elif name in ("batched48", "batched48_bicgstab", "sloppy48"):
@@ -628,7 +672,8 @@ def profile_case(w, name):
         else:
             fb = ctx.field(nrhs=n, data=srcs)
         del A
-        R = 1 if n == 1 else 2
+        if R is None:
+            R = 1 if n == 1 else 2
         if name == "sloppy48":
             fin = ctx.field(prec=w.F16, nrhs=n)
             fin.copy_from(fb)
@@ -638,18 +683,19 @@ def profile_case(w, name):
             for _ in range(10):
                 ctx.apply_M_sloppy(fout, fin, Us, masses, False, R, 0)
             ctx.sync()
-            out = {"case": name, "launches": 10,
+            out = {"case": name, "launches": 10, "rhs_per_thread": R,
                    "ms_per_launch": round((time.perf_counter() - t1) * 100, 4)}
         else:
             x = ctx.field(nrhs=n)
-            if name == "batched48_bicgstab":
+            if name in ("batched48_bicgstab", "bicgstab32"):
                 res = ctx.bicgstab_mixed(x, fb, U, Us, masses, tol=1e-10, delta=DELTA,
                                          rhs_per_thread=R, site_chunk=0)
             else:
                 res = ctx.cg_mixed(x, fb, U, Us, masses, tol=1e-10, delta=DELTA,
                                    rhs_per_thread=R, site_chunk=0, even_odd=name != "mixed32")
             require_converged(res, name)
-            out = {"case": name, "systems": n, "seconds": round(res["seconds"], 4),
+            out = {"case": name, "systems": n, "rhs_per_thread": R,
+                   "seconds": round(res["seconds"], 4),
                    "iterations": max(s["iterations"] for s in res["systems"]),
                    "max_true_rel_residual": max(s["true_rel_rsd"] for s in res["systems"])}
     out["lattice"] = L
@@ -679,13 +725,16 @@ def main():
     ap.add_argument("--cpu-worker", default=None)
     ap.add_argument("--profile-case", choices=PROFILE_CASES, default=None,
                     help="run one solve or operator case for rocprofv3 and exit")
+    ap.add_argument("--profile-r", type=int, default=None,
+                    help="systems per thread of the profiled case (1, 2, 4 or 64 = every "
+                         "system of a site in one thread)")
     args = ap.parse_args()
 
     if args.cpu_worker:
         cpu_baseline_worker(args.lattice, args.cpu_worker, args.cpu_seconds)
         return
     if args.profile_case:
-        profile_case(load_pkg().wilson_hip, args.profile_case)
+        profile_case(load_pkg().wilson_hip, args.profile_case, args.profile_r)
         return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; multi-GPU runs (--gpus > 1) are not "

@@ -179,12 +179,19 @@ int wd_cg_eo(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd_gauge *U, dou
  * F16 (per-site scaled) or F32 spinors and a sloppy gauge copy (F32 18/12-real
  * or F16 12-real); FP64 is kept for reductions, the solution and the true
  * residual of the reliable updates. */
+/* rhs_per_thread value: every system of a site in one thread, one after the
+ * other, with the site's links loaded once and kept in registers as packed
+ * halves; SU(3) products by packed F16 dot products accumulated in FP32 (the
+ * half spinor and the rebuilt third link row rounded to F16).  F16 spinors and
+ * F16 12-real links only.  Results differ from R = 1, 2, 4 by those roundings. */
+#define WD_RHS_LOOP 64
+
 typedef struct wd_mixed_params {
     int max_iter;       /* inner iterations per system */
     double tol;         /* stop when the FP64 |M^dag b - M^dag M x| < tol |M^dag b| */
     double delta;       /* reliable update when the sloppy |r| < delta x |r| at the last one */
     int spinor_prec;    /* WD_F16 or WD_F32 storage of the inner vectors */
-    int rhs_per_thread; /* 1, 2 or 4 systems per thread; 0 = autotune */
+    int rhs_per_thread; /* 1, 2 or 4 systems per thread, WD_RHS_LOOP; 0 = autotune */
     int site_chunk;     /* site blocks (256 sites) per L2 chunk of the multi-system
                            operator; 0 = off, -1 = autotune (with rhs_per_thread) */
     int check_every;    /* iterations between host polls of the convergence flags */
@@ -238,10 +245,19 @@ int wd_bicgstab_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd_gaug
 
 /* out_r = M_r in_r (dagger 0) or M_r^dag in_r with the sloppy operator for all
  * nrhs systems (fields F16 or F32 of the same format, masses[r] host array),
- * rhs_per_thread 1, 2 or 4, site_chunk >= 0 (work order only: every chunk
- * gives bit-identical results). */
+ * rhs_per_thread 1, 2, 4 or WD_RHS_LOOP, site_chunk >= 0 (work order only:
+ * every chunk gives bit-identical results). */
 int wd_apply_M_sloppy(wd_ctx *ctx, wd_field *out, const wd_field *in, const wd_gauge *Us,
                       const double *masses, int dagger, int rhs_per_thread, int site_chunk);
+
+/* Average duration of the sloppy M over in's systems (cb = -1, both parities)
+ * or of its one-parity launch (cb = 0 or 1, the shape of the Schur solvers'
+ * operator), iters launches timed with HIP events on the context stream after
+ * one untimed launch; rhs_per_thread 1, 2, 4 or WD_RHS_LOOP.  No counterpart in
+ * the reference (measurement). */
+int wd_time_sloppy(wd_ctx *ctx, wd_field *out, const wd_field *in, const wd_gauge *Us,
+                   const double *masses, int rhs_per_thread, int site_chunk, int cb, int iters,
+                   double *ms_per_launch);
 
 /* Candidates of the autotuner: rhs per thread {1, 2, 4} x site chunk
  * {0, 16, 32, 64}. */
@@ -320,6 +336,11 @@ int wd_time_op(wd_ctx *ctx, int op, wd_field *out, const wd_field *in, const wd_
  * once for both (links are three quarters of an FP64 hop's bytes); 0 = one
  * system per thread.  Results are bit-identical. */
 #define WD_OPT_F64_PAIRS 5
+/* The WD_RHS_LOOP sloppy kernel's neighbour prefetch: 3 (default) = the
+ * loads of the third next hop direction are issued before each direction's
+ * arithmetic, at 2 wavefronts per SIMD; 1 = the next direction's, at 3
+ * wavefronts per SIMD.  Results are bit-identical. */
+#define WD_OPT_LOOP_PREFETCH 6
 int wd_set_option(wd_ctx *ctx, int option, int value);
 /* The value an option has in effect on this context: 0/1 for
  * WD_OPT_XCD_REMAP, WD_OPT_PARITY_ORDER and WD_OPT_F64_PAIRS, the column width W (0 = natural

@@ -999,8 +999,12 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
     WD_ARG_CHECK(ctx, wd_sloppy_supported(prm->spinor_prec, Us->prec, Us->recon),
                  "mixed cg: unsupported sloppy link format");
     WD_ARG_CHECK(ctx, prm->rhs_per_thread == 0 || prm->rhs_per_thread == 1 ||
-                          prm->rhs_per_thread == 2 || prm->rhs_per_thread == 4,
-                 "mixed cg: rhs_per_thread must be 0, 1, 2 or 4");
+                          prm->rhs_per_thread == 2 || prm->rhs_per_thread == 4 ||
+                          prm->rhs_per_thread == WD_RHS_LOOP,
+                 "mixed cg: rhs_per_thread must be 0, 1, 2, 4 or WD_RHS_LOOP");
+    WD_ARG_CHECK(ctx, prm->rhs_per_thread != WD_RHS_LOOP ||
+                          wd_sloppy_loop_supported(prm->spinor_prec, Us->prec, Us->recon),
+                 "mixed cg: WD_RHS_LOOP needs F16 spinors and F16 12-real links");
     WD_ARG_CHECK(ctx, prm->site_chunk >= -1, "mixed cg: site_chunk must be >= -1");
     WD_ARG_CHECK(ctx, prm->max_iter >= 1 && prm->check_every >= 1 && prm->tol > 0.0 &&
                           prm->delta > 0.0 && prm->delta < 1.0 && prm->reliable_every >= 0,
@@ -1027,8 +1031,9 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
     const int nel = eo ? stride : 2 * stride;
     const int eg = (nel + kBlock - 1) / kBlock;
     const int bpp = (int)((ctx->Vh + kBlock - 1) / kBlock);
-    const int hop_nsb = eo ? bpp : 2 * bpp; // partials of the (last) operator launch
-    const int nsb_max = std::max(eg_full, 2 * bpp);
+    const int hop_nsb = eo ? bpp : 2 * bpp; // workgroups of the (last) operator launch
+    // room for the operator's partials at kLoopWaves per workgroup (WD_RHS_LOOP)
+    const int nsb_max = std::max(eg_full, 2 * bpp * kLoopWaves);
     DevBuf partials, state, flags, diag_d, diag_f, coef;
     if ((rc = partials.alloc(ctx, sizeof(double2) * (size_t)n * nsb_max)) ||
         (rc = state.alloc(ctx, sizeof(MxState) * n)) ||
@@ -1154,7 +1159,7 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
             C = tC;
     }
     A.chunk = C;
-    const int nsb = hop_nsb;
+    const int nsb = hop_nsb * wd_sloppy_partials_per_block(R); // partials of the operator
 
     // sloppy out = M in (full) or A in (even-odd), with the partials of
     // (dotv, out) for HOP_M_DOT
@@ -1425,8 +1430,12 @@ extern "C" int wd_bicgstab_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, co
     WD_ARG_CHECK(ctx, wd_sloppy_supported(prm->spinor_prec, Us->prec, Us->recon),
                  "mixed bicgstab: unsupported sloppy link format");
     WD_ARG_CHECK(ctx, prm->rhs_per_thread == 0 || prm->rhs_per_thread == 1 ||
-                          prm->rhs_per_thread == 2 || prm->rhs_per_thread == 4,
-                 "mixed bicgstab: rhs_per_thread must be 0, 1, 2 or 4");
+                          prm->rhs_per_thread == 2 || prm->rhs_per_thread == 4 ||
+                          prm->rhs_per_thread == WD_RHS_LOOP,
+                 "mixed bicgstab: rhs_per_thread must be 0, 1, 2, 4 or WD_RHS_LOOP");
+    WD_ARG_CHECK(ctx, prm->rhs_per_thread != WD_RHS_LOOP ||
+                          wd_sloppy_loop_supported(prm->spinor_prec, Us->prec, Us->recon),
+                 "mixed bicgstab: WD_RHS_LOOP needs F16 spinors and F16 12-real links");
     WD_ARG_CHECK(ctx, prm->site_chunk >= -1, "mixed bicgstab: site_chunk must be >= -1");
     WD_ARG_CHECK(ctx, prm->max_iter >= 1 && prm->check_every >= 1 && prm->tol > 0.0 &&
                           prm->delta > 0.0 && prm->delta < 1.0 && prm->reliable_every >= 0,
@@ -1451,8 +1460,8 @@ extern "C" int wd_bicgstab_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, co
     const int nel = stride; // the even half
     const int eg = (nel + kBlock - 1) / kBlock;
     const int bpp = (int)((ctx->Vh + kBlock - 1) / kBlock);
-    const int hop_nsb = bpp; // partials of a one-parity operator launch
-    const int nsb_max = std::max(eg_full, 2 * bpp);
+    const int hop_nsb = bpp; // workgroups of a one-parity operator launch
+    const int nsb_max = std::max(eg_full, 2 * bpp * kLoopWaves);
     DevBuf pa_buf, pb_buf, state, flags, diag_d, diag_f, coef;
     if ((rc = pa_buf.alloc(ctx, sizeof(double2) * (size_t)n * nsb_max)) ||
         (rc = pb_buf.alloc(ctx, sizeof(double2) * (size_t)n * nsb_max)) ||
@@ -1555,6 +1564,7 @@ extern "C" int wd_bicgstab_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, co
             C = tC;
     }
     As.chunk = C;
+    const int op_parts = hop_nsb * wd_sloppy_partials_per_block(R); // partials of the operator
     // norm: also the partials of |out|^2 into pb
     auto apply_A = [&](wd_field *out, const wd_field *in, const wd_field *dotv, bool norm) -> int {
         SloppyArgs m = As;
@@ -1602,14 +1612,14 @@ extern "C" int wd_bicgstab_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, co
         if ((e = apply_A(v.f, pv.f, r0.f, false)))
             return e;
         hipLaunchKernelGGL(k_bs_alpha, dim3(n), dim3(1024), 0, ctx->stream, (const double2 *)pa,
-                           hop_nsb, st, active);
+                           op_parts, st, active);
         BS_SF(k_bs_s, (char *)sv.f->data, sv.f->scale, (const char *)r.f->data,
               (const float *)r.f->scale, (const char *)v.f->data, (const float *)v.f->scale, srb,
               stride, nel, (const BsState *)st, (const int *)active);
         if ((e = apply_A(t.f, sv.f, sv.f, true)))   // t = A s, (s, t) into pa, |t|^2 into pb
             return e;
         hipLaunchKernelGGL(k_bs_omega, dim3(n), dim3(1024), 0, ctx->stream, (const double2 *)pa,
-                           hop_nsb, (const double2 *)pb, hop_nsb, st, active);
+                           op_parts, (const double2 *)pb, op_parts, st, active);
         BS_SF(k_bs_update, (char *)xs.f->data, (char *)r.f->data, r.f->scale,
               (const char *)pv.f->data, (const float *)pv.f->scale, (const char *)sv.f->data,
               (const float *)sv.f->scale, (const char *)t.f->data, (const float *)t.f->scale,
@@ -1805,18 +1815,20 @@ extern "C" int wd_tune_report(wd_ctx *ctx, const wd_field *f, const wd_gauge *Us
     return rc;
 }
 
-extern "C" int wd_apply_M_sloppy(wd_ctx *ctx, wd_field *out, const wd_field *in,
-                                 const wd_gauge *Us, const double *masses, int dagger,
-                                 int rhs_per_thread, int site_chunk)
+// Launches the sloppy M (cb = -1, both parities) or its one-parity part
+// (cb = 0 or 1, the launch shape of the Schur solvers) `iters` times; with ms
+// the average launch duration from HIP events on the context stream.
+static int sloppy_apply(wd_ctx *ctx, wd_field *out, const wd_field *in, const wd_gauge *Us,
+                        const double *masses, int dagger, int rhs_per_thread, int site_chunk,
+                        int cb, int iters, double *ms)
 {
-    if (!ctx)
-        return WD_EARG;
     WD_ARG_CHECK(ctx, out && in && Us && masses, "apply_M_sloppy: null argument");
     WD_ARG_CHECK(ctx, out != in, "apply_M_sloppy: out and in must be distinct fields");
     WD_ARG_CHECK(ctx, out->prec == in->prec && out->nrhs == in->nrhs,
                  "apply_M_sloppy: out and in must have the same format and nrhs");
     WD_ARG_CHECK(ctx, wd_sloppy_supported(in->prec, Us->prec, Us->recon),
                  "apply_M_sloppy: unsupported precision combination");
+    WD_ARG_CHECK(ctx, cb >= -1 && cb <= 1 && iters >= 1, "time_sloppy: bad cb or iteration count");
     const int n = in->nrhs;
     DevBuf diag;
     int rc = diag.alloc(ctx, sizeof(float) * n);
@@ -1838,10 +1850,56 @@ extern "C" int wd_apply_M_sloppy(wd_ctx *ctx, wd_field *out, const wd_field *in,
     a.nrhs = n;
     a.hc = -0.5f;
     a.chunk = site_chunk;
-    rc = wd_launch_hop_sloppy(ctx, a, in->prec, Us->prec, Us->recon, rhs_per_thread, dagger, -1,
-                              HOP_M);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (ms) {
+        // one untimed launch first (argument checks, first use of the kernel)
+        if ((rc = wd_launch_hop_sloppy(ctx, a, in->prec, Us->prec, Us->recon, rhs_per_thread,
+                                       dagger, cb, HOP_M)))
+            return rc;
+        WD_HIP_CHECK(ctx, hipEventCreate(&e0));
+        if (hipEventCreate(&e1) != hipSuccess) {
+            (void)hipEventDestroy(e0);
+            return wd_set_error(ctx, WD_EHIP, "time_sloppy: event creation failed");
+        }
+        (void)hipEventRecord(e0, ctx->stream);
+    }
+    for (int k = 0; k < iters && !rc; ++k)
+        rc = wd_launch_hop_sloppy(ctx, a, in->prec, Us->prec, Us->recon, rhs_per_thread, dagger,
+                                  cb, HOP_M);
+    if (ms) {
+        hipError_t er = hipEventRecord(e1, ctx->stream);
+        hipError_t es = er == hipSuccess ? hipEventSynchronize(e1) : er;
+        float t = 0.f;
+        hipError_t ee = es == hipSuccess ? hipEventElapsedTime(&t, e0, e1) : es;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (!rc && ee != hipSuccess)
+            rc = wd_set_error(ctx, WD_EHIP, std::string("time_sloppy: ") + hipGetErrorString(ee));
+        *ms = (double)t / iters;
+    }
     if (rc)
         return rc;
     WD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     return WD_OK;
+}
+
+extern "C" int wd_apply_M_sloppy(wd_ctx *ctx, wd_field *out, const wd_field *in,
+                                 const wd_gauge *Us, const double *masses, int dagger,
+                                 int rhs_per_thread, int site_chunk)
+{
+    if (!ctx)
+        return WD_EARG;
+    return sloppy_apply(ctx, out, in, Us, masses, dagger, rhs_per_thread, site_chunk, -1, 1,
+                        nullptr);
+}
+
+extern "C" int wd_time_sloppy(wd_ctx *ctx, wd_field *out, const wd_field *in, const wd_gauge *Us,
+                              const double *masses, int rhs_per_thread, int site_chunk, int cb,
+                              int iters, double *ms_per_launch)
+{
+    if (!ctx)
+        return WD_EARG;
+    WD_ARG_CHECK(ctx, ms_per_launch != nullptr, "time_sloppy: null result");
+    return sloppy_apply(ctx, out, in, Us, masses, 0, rhs_per_thread, site_chunk, cb, iters,
+                        ms_per_launch);
 }

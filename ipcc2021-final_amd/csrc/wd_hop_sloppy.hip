@@ -32,46 +32,12 @@
 #include "wd_fields.h"
 #include "wd_internal.h"
 #include "wd_sloppy.h"
+#include "wd_sloppy_dev.h"
 
 using namespace wdf;
+using namespace wdsl;
 
 namespace {
-
-// Byte offset of 16-byte chunk q of site j in a [chunks][stride] array; the
-// launcher checks that every field fits 32-bit offsets, so loads use a
-// uniform 64-bit base plus a 32-bit lane offset.
-__device__ __forceinline__ unsigned off16(int q, int stride, unsigned j)
-{
-    return ((unsigned)q * (unsigned)stride + j) * 16u;
-}
-
-// The 12 packed complex of an F16 site (component k = 3 * spin + colour).
-__device__ __forceinline__ void load16(const char *base, int stride, unsigned j, unsigned (&w)[12])
-{
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        const u4v c = *reinterpret_cast<const u4v *>(base + off16(q, stride, j));
-        w[4 * q] = c.x;
-        w[4 * q + 1] = c.y;
-        w[4 * q + 2] = c.z;
-        w[4 * q + 3] = c.w;
-    }
-}
-
-__device__ __forceinline__ void load32(const char *base, int stride, unsigned j, cpx<float> (&v)[4][3])
-{
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-        const f4v c = *reinterpret_cast<const f4v *>(base + off16(q, stride, j));
-        v[(2 * q) / 3][(2 * q) % 3] = {c.x, c.y};
-        v[(2 * q + 1) / 3][(2 * q + 1) % 3] = {c.z, c.w};
-    }
-}
-
-__device__ __forceinline__ float load_scale(const float *base, unsigned j)
-{
-    return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + j * 4u);
-}
 
 // a + mulc<P>(b) from two packed complex, one v_fma_mix_f32 per real.  `one`
 // is 1.0f passed in from the host: with a literal 1 the compiler would fold
@@ -87,40 +53,6 @@ __device__ __forceinline__ cpx<float> proj16(unsigned wa, unsigned wb, float one
         return {fmaf(im16(wb), -one, re16(wa)), fmaf(re16(wb), one, im16(wa))};
     else // -i b = (b.i, -b.r)
         return {fmaf(im16(wb), one, re16(wa)), fmaf(re16(wb), -one, im16(wa))};
-}
-
-// acc += f * mulc<C>(z)
-template <int C>
-__device__ __forceinline__ void acc_mulc(cpx<float> &acc, cpx<float> z, float f)
-{
-    if constexpr (C == 0) {
-        acc.r = fmaf(f, z.r, acc.r);
-        acc.i = fmaf(f, z.i, acc.i);
-    } else if constexpr (C == 1) {
-        acc.r = fmaf(-f, z.r, acc.r);
-        acc.i = fmaf(-f, z.i, acc.i);
-    } else if constexpr (C == 2) {
-        acc.r = fmaf(-f, z.i, acc.r);
-        acc.i = fmaf(f, z.r, acc.i);
-    } else {
-        acc.r = fmaf(f, z.i, acc.r);
-        acc.i = fmaf(-f, z.r, acc.i);
-    }
-}
-
-// acc += f * reconstruct(r) for direction D; f carries -1/2, the boundary
-// phase and the F16 site scale, so the reconstruction is 24 FMAs.
-template <class D>
-__device__ __forceinline__ void reconstruct_f(cpx<float> (&acc)[4][3], const cpx<float> (&r)[2][3],
-                                              float f)
-{
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        acc_mulc<0>(acc[0][c], r[0][c], f);
-        acc_mulc<0>(acc[1][c], r[1][c], f);
-        acc_mulc<D::R0>(acc[D::S0][c], r[0][c], f);
-        acc_mulc<D::R1>(acc[D::S1][c], r[1][c], f);
-    }
 }
 
 // One link with 32-bit lane offsets (row 2 rebuilt for 12-real formats).  NT:
@@ -213,77 +145,6 @@ __device__ __forceinline__ void dir_rhs(cpx<float> (&acc)[R][4][3], const Sloppy
         reconstruct_f<D>(acc[r], res, f);
     }
     pin_accs<R>(acc);
-}
-
-// Block-wide sum of (sr, si) stored by thread 0 to dst; safe to call repeatedly.
-__device__ __forceinline__ void block_sum_store(double sr, double si, double2 *dst)
-{
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        sr += __shfl_xor(sr, off, 64);
-        si += __shfl_xor(si, off, 64);
-    }
-    __shared__ double lr[kBlock / 64], li[kBlock / 64];
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
-        lr[w] = sr;
-        li[w] = si;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double tr = 0.0, ti = 0.0;
-        for (int k = 0; k < kBlock / 64; ++k) {
-            tr += lr[k];
-            ti += li[k];
-        }
-        *dst = make_double2(tr, ti);
-    }
-    __syncthreads();
-}
-
-// Stores an F16 site (values / max |component|, scale = that max) and returns
-// the packed words and the scale as stored.
-__device__ __forceinline__ float store16(char *base, float *scale, int stride, unsigned i,
-                                         const cpx<float> (&v)[4][3], unsigned (&w)[12])
-{
-    float m = 0.f;
-#pragma unroll
-    for (int k = 0; k < 12; ++k)
-        m = fmaxf(m, fmaxf(fabsf(v[k / 3][k % 3].r), fabsf(v[k / 3][k % 3].i)));
-    // below 1e-30 the site is stored as zero (1/m would overflow for denormal
-    // m, turning zeros into NaN); NaN inputs still propagate
-    const bool tiny = m < 1.0e-30f;
-    const float inv = tiny ? 0.f : 1.0f / m;
-    const float s = tiny ? 0.f : m;
-    // outputs are streamed (non-temporal) so they do not evict from L2 the
-    // input spinors that neighbouring workgroups still read
-    __builtin_nontemporal_store(s, reinterpret_cast<float *>(reinterpret_cast<char *>(scale) + i * 4u));
-#pragma unroll
-    for (int k = 0; k < 12; ++k)
-        w[k] = f2h(v[k / 3][k % 3].r * inv, v[k / 3][k % 3].i * inv);
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        u4v c;
-        c.x = w[4 * q];
-        c.y = w[4 * q + 1];
-        c.z = w[4 * q + 2];
-        c.w = w[4 * q + 3];
-        __builtin_nontemporal_store(c, reinterpret_cast<u4v *>(base + off16(q, stride, i)));
-    }
-    return s;
-}
-
-__device__ __forceinline__ void store32(char *base, int stride, unsigned i, const cpx<float> (&v)[4][3])
-{
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-        f4v c;
-        c.x = v[(2 * q) / 3][(2 * q) % 3].r;
-        c.y = v[(2 * q) / 3][(2 * q) % 3].i;
-        c.z = v[(2 * q + 1) / 3][(2 * q + 1) % 3].r;
-        c.w = v[(2 * q + 1) / 3][(2 * q + 1) % 3].i;
-        __builtin_nontemporal_store(c, reinterpret_cast<f4v *>(base + off16(q, stride, i)));
-    }
 }
 
 template <int SF, int GF, int R, bool DAG, int MODE>
@@ -569,7 +430,12 @@ bool wd_sloppy_supported(int spinor_prec, int gauge_prec, int recon)
     return s && g;
 }
 
-int wd_sloppy_groups(int nrhs, int R) { return (nrhs + R - 1) / R; }
+bool wd_sloppy_loop_supported(int spinor_prec, int gauge_prec, int recon)
+{
+    return spinor_prec == WD_F16 && gauge_prec == WD_F16 && recon == WD_RECON_12;
+}
+
+int wd_sloppy_groups(int nrhs, int R) { return R == WD_RHS_LOOP ? 1 : (nrhs + R - 1) / R; }
 
 int wd_launch_hop_sloppy(wd_ctx *ctx, SloppyArgs a, int spinor_prec, int gauge_prec, int recon,
                          int R, int dagger, int cb, int mode)
@@ -579,7 +445,10 @@ int wd_launch_hop_sloppy(wd_ctx *ctx, SloppyArgs a, int spinor_prec, int gauge_p
     Geom g = ctx->geom;
     if (g.tW > 0)
         g.tW = 2 * g.tW < g.tD ? 2 * g.tW : 0;
-    WD_ARG_CHECK(ctx, R == 1 || R == 2 || R == 4, "sloppy hop: rhs per thread must be 1, 2 or 4");
+    WD_ARG_CHECK(ctx, R == 1 || R == 2 || R == 4 || R == WD_RHS_LOOP,
+                 "sloppy hop: rhs per thread must be 1, 2, 4 or WD_RHS_LOOP");
+    WD_ARG_CHECK(ctx, R != WD_RHS_LOOP || wd_sloppy_loop_supported(spinor_prec, gauge_prec, recon),
+                 "sloppy hop: WD_RHS_LOOP needs F16 spinors and F16 12-real links");
     WD_ARG_CHECK(ctx, a.chunk >= 0, "sloppy hop: site chunk must be >= 0");
     // every per-parity array (at most 6 x 16 bytes per site) must fit 32-bit offsets
     WD_ARG_CHECK(ctx, 96.0 * g.stride < 4294967296.0, "sloppy hop: lattice too large");
@@ -596,6 +465,8 @@ int wd_launch_hop_sloppy(wd_ctx *ctx, SloppyArgs a, int spinor_prec, int gauge_p
     a.tlinks = a.groups > 1 || (nparity == 2 && ctx->parity_il);
     const long grid = (long)a.nsb * a.groups;
     WD_ARG_CHECK(ctx, grid < (1L << 31), "sloppy hop: grid too large");
+    if (R == WD_RHS_LOOP)
+        return wd_launch_hop_sloppy_loop(ctx, a, g, dagger, cb, mode);
     const int cb_first = (cb < 0) ? 0 : cb;
     const int gf = gauge_fmt(gauge_prec, recon);
     if (spinor_prec == WD_F16) {

@@ -30,7 +30,16 @@ struct SloppyArgs {
 };
 
 bool wd_sloppy_supported(int spinor_prec, int gauge_prec, int recon);
+// formats of the kernel that loops over every system of a site (WD_RHS_LOOP)
+bool wd_sloppy_loop_supported(int spinor_prec, int gauge_prec, int recon);
 int wd_sloppy_groups(int nrhs, int R);
-// cb in {0, 1}: one output parity; cb = -1: both parities.  R in {1, 2, 4}.
+// cb in {0, 1}: one output parity; cb = -1: both parities.  R in {1, 2, 4} or
+// WD_RHS_LOOP (wd_hop_sloppy_loop.hip).
 int wd_launch_hop_sloppy(wd_ctx *ctx, SloppyArgs a, int spinor_prec, int gauge_prec, int recon,
                          int R, int dagger, int cb, int mode);
+int wd_launch_hop_sloppy_loop(wd_ctx *ctx, const SloppyArgs &a, const Geom &g, int dagger, int cb,
+                              int mode);
+// partial sums per workgroup of the HOP_M_DOT / HOP_M_DOT_NORM launches: the
+// WD_RHS_LOOP kernel writes one per wavefront, the others one per workgroup
+constexpr int kLoopWaves = kBlock / 64;
+inline int wd_sloppy_partials_per_block(int R) { return R == WD_RHS_LOOP ? kLoopWaves : 1; }

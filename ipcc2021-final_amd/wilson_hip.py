@@ -28,11 +28,13 @@ WD_ENOTCONV = 1
 
 F64, F32, F16 = 0, 1, 2
 OP_HOP, OP_M = 0, 1
+RHS_LOOP = 64   # rhs_per_thread: every system of a site in one thread (F16 spinors and links)
 OPT_XCD_REMAP = 1
 OPT_BLOCK_TILE = 2
 OPT_PARITY_ORDER = 3
 OPT_TSTREAM = 4
 OPT_F64_PAIRS = 5
+OPT_LOOP_PREFETCH = 6
 RECON_18, RECON_12 = 18, 12
 # autotuner candidates (include/wilson.h WD_TUNE_NR x WD_TUNE_NC)
 TUNE_R = (1, 2, 4)
@@ -51,7 +53,7 @@ EXPORTED = [
     "wd_cg_params_default", "wd_cg", "wd_cg_eo", "wd_time_op", "wd_set_option",
     "wd_get_option",
     "wd_mixed_params_default", "wd_cg_mixed", "wd_bicgstab_mixed", "wd_apply_M_sloppy",
-    "wd_tune_report",
+    "wd_tune_report", "wd_time_sloppy",
     "wd_pion_correlator", "wd_pion_correlator_noalloc",
     "wdg_gauge_random_su3", "wdg_gauge_near_identity", "wdg_gauge_unit",
     "wdg_gauge_flip_boundary", "wdg_spinor_gaussian", "wdg_spinor_point", "wdg_philox4x32",
@@ -162,6 +164,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
                                ctypes.POINTER(MixedReport), ctypes.POINTER(d)], i),
         "wd_apply_M_sloppy": ([vp, vp, vp, vp, dp, i, i, i], i),
         "wd_tune_report": ([vp, vp, vp, ctypes.POINTER(i), ctypes.POINTER(i), dp], i),
+        "wd_time_sloppy": ([vp, vp, vp, vp, dp, i, i, i, i, ctypes.POINTER(d)], i),
         "wd_time_op": ([vp, i, vp, vp, vp, d, i, i, i, ctypes.POINTER(d)], i),
         "wd_pion_correlator": ([vp, vp, i, i, i, dp], i),
         "wd_pion_correlator_noalloc": ([vp, vp, i, i, i, dp], i),
@@ -341,6 +344,18 @@ class Context:
         assert m.size == inp.nrhs
         self.check(self.lib.wd_apply_M_sloppy(self.h, out.h, inp.h, Us.h, _ptr(m), int(dagger),
                                               rhs_per_thread, site_chunk), "wd_apply_M_sloppy")
+
+    def time_sloppy(self, out, inp, Us, masses, rhs_per_thread: int = 1, site_chunk: int = 0,
+                    cb: int = -1, iters: int = 10) -> float:
+        """Average launch duration in ms of the sloppy M (cb = -1) or its
+        one-parity launch (cb = 0, 1), HIP events on the context stream."""
+        m = np.ascontiguousarray(masses, dtype=np.float64)
+        assert m.size == inp.nrhs
+        ms = ctypes.c_double()
+        self.check(self.lib.wd_time_sloppy(self.h, out.h, inp.h, Us.h, _ptr(m), rhs_per_thread,
+                                           site_chunk, cb, iters, ctypes.byref(ms)),
+                   "wd_time_sloppy")
+        return ms.value
 
     def tune_report(self, f, Us):
         """Times every (rhs per thread, site chunk) candidate of the sloppy M over

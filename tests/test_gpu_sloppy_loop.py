@@ -1,0 +1,132 @@
+"""The sloppy operator with every system of a site in one thread
+(rhs_per_thread = WD_RHS_LOOP: links loaded once per site and kept as packed
+halves, SU(3) products by packed F16 dot products with FP32 accumulation).
+
+Tolerances: against the FP64 oracle the same 2e-3 relative L2 as the other
+F16-spinor formats; against the R = 1 kernel (FP32 products of the same F16
+inputs) 1e-3, the size of the extra F16 roundings of the half spinor and the
+rebuilt third link row.  Solvers running on it must still meet the FP64 stop
+test 1e-10 (re-checked with the oracle), every system on its own iteration
+count (per-system masking), on lattices whose half volume is not a multiple
+of the workgroup (idle lanes).
+"""
+import numpy as np
+import pytest
+
+from oracle_py import kappa_to_mass, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(w, L, nrhs, seed=20211025):
+    A = w.gauge_near_identity(L, seed, 0.3)
+    bs = [w.spinor_gaussian(L, seed + 1, r) for r in range(nrhs)]
+    return A, bs
+
+
+@pytest.mark.parametrize("L", [[4, 4, 6, 8], [6, 4, 6, 10]], ids=lambda L: "x".join(map(str, L)))
+@pytest.mark.parametrize("dag", [0, 1])
+def test_loop_operator_matches_oracle(w, oracle, L, dag):
+    n = 5
+    A, bs = _setup(w, L, n)
+    masses = [kappa_to_mass(k) for k in (0.12, 0.125, 0.13, 0.135, 0.11)]
+    with w.Context(L, bc=[1, 1, 1, -1]) as ctx:
+        Us = ctx.gauge(A, prec=w.F16, recon=12)
+        fin = ctx.field(prec=w.F16, nrhs=n, data=bs)
+        fl, f1 = ctx.field(prec=w.F16, nrhs=n), ctx.field(prec=w.F16, nrhs=n)
+        ctx.apply_M_sloppy(fl, fin, Us, masses, bool(dag), w.RHS_LOOP)
+        ctx.apply_M_sloppy(f1, fin, Us, masses, bool(dag), 1)
+        got = [fl.download(r) for r in range(n)]
+        ref1 = [f1.download(r) for r in range(n)]
+    Af = [a.copy() for a in A]
+    w.gauge_flip_boundary(L, 3, Af)
+    for r in range(n):
+        ref = oracle.dslash(bs[r], Af, L, masses[r], dag)
+        e_or, e_r1 = rel_l2(got[r], ref), rel_l2(got[r], ref1[r])
+        print(f"L={L} dag={dag} system {r}: vs oracle {e_or:.2e} (R=1: "
+              f"{rel_l2(ref1[r], ref):.2e}), vs R=1 {e_r1:.2e}", flush=True)
+        assert e_or < 2e-3 and e_r1 < 1e-3
+
+
+@pytest.mark.parametrize("solver", ["cg_mixed", "bicgstab_mixed"])
+def test_loop_solvers_reach_fp64_residual(w, oracle, solver):
+    L = [6, 4, 6, 10]
+    n = 6
+    A, bs = _setup(w, L, n)
+    masses = [kappa_to_mass(k) for k in (0.11, 0.12, 0.125, 0.13, 0.135, 0.12)]
+    bs[5] = np.zeros_like(bs[5])   # solved by x = 0 without iterating
+    with w.Context(L, bc=[1, 1, 1, -1]) as ctx:
+        U = ctx.gauge(A)
+        Us = ctx.gauge(A, prec=w.F16, recon=12)
+        fb = ctx.field(nrhs=n, data=bs)
+        x = ctx.field(nrhs=n)
+        kw = {"even_odd": True} if solver == "cg_mixed" else {}
+        res = getattr(ctx, solver)(x, fb, U, Us, masses, tol=1e-10, rhs_per_thread=w.RHS_LOOP,
+                                   check_every=4, **kw)
+        xs = [x.download(r) for r in range(n)]
+    its = [s["iterations"] for s in res["systems"]]
+    print(solver, its, [s["rhs_per_thread"] for s in res["systems"]], flush=True)
+    assert res["status"] == 0
+    assert all(s["rhs_per_thread"] == w.RHS_LOOP for s in res["systems"])
+    assert its[0] < its[4] and its[5] == 0 and np.all(xs[5] == 0)
+    Af = [a.copy() for a in A]
+    w.gauge_flip_boundary(L, 3, Af)
+    for r in range(5):
+        Mdb = oracle.dslash(bs[r], Af, L, masses[r], 1)
+        rr = Mdb - oracle.dslash(oracle.dslash(xs[r], Af, L, masses[r], 0), Af, L, masses[r], 1)
+        assert res["systems"][r]["converged"]
+        assert np.linalg.norm(rr) / np.linalg.norm(Mdb) < 1.05e-10
+
+
+def test_loop_graph_and_direct_launch_agree(w):
+    L = [4, 4, 4, 8]
+    A, bs = _setup(w, L, 3)
+    masses = [0.1, 0.2, -0.1]
+    out = []
+    with w.Context(L) as ctx:
+        U = ctx.gauge(A)
+        Us = ctx.gauge(A, prec=w.F16, recon=12)
+        fb = ctx.field(nrhs=3, data=bs)
+        for g in (True, False):
+            x = ctx.field(nrhs=3)
+            res = ctx.bicgstab_mixed(x, fb, U, Us, masses, use_graph=g, check_every=3,
+                                     rhs_per_thread=w.RHS_LOOP)
+            out.append(([x.download(r) for r in range(3)], res))
+    for r in range(3):
+        assert np.array_equal(out[0][0][r], out[1][0][r])
+
+
+def test_loop_argument_errors(w):
+    L = [4, 4, 4, 4]
+    with w.Context(L) as ctx:
+        U32 = ctx.gauge(w.gauge_unit(L), prec=w.F32, recon=12)
+        f16 = ctx.field(prec=w.F16, nrhs=2)
+        o16 = ctx.field(prec=w.F16, nrhs=2)
+        with pytest.raises(w.WilsonError) as e:   # F32 links: not a loop-kernel format
+            ctx.apply_M_sloppy(o16, f16, U32, [0.1, 0.1], rhs_per_thread=w.RHS_LOOP)
+        assert e.value.code == w.WD_EARG
+        U = ctx.gauge(w.gauge_unit(L))
+        x, b = ctx.field(nrhs=2), ctx.field(nrhs=2)
+        with pytest.raises(w.WilsonError) as e:
+            ctx.cg_mixed(x, b, U, U32, [0.1, 0.1], rhs_per_thread=w.RHS_LOOP)
+        assert e.value.code == w.WD_EARG
+
+
+def test_time_sloppy(w):
+    """wd_time_sloppy: positive launch times for both kernels and both launch
+    shapes, argument errors before any launch."""
+    L = [8, 8, 8, 8]
+    A, bs = _setup(w, L, 4)
+    masses = [0.1, 0.2, 0.3, 0.0]
+    with w.Context(L) as ctx:
+        Us = ctx.gauge(A, prec=w.F16, recon=12)
+        fin = ctx.field(prec=w.F16, nrhs=4, data=bs)
+        fout = ctx.field(prec=w.F16, nrhs=4)
+        for R in (1, 2, w.RHS_LOOP):
+            for cb in (-1, 0, 1):
+                ms = ctx.time_sloppy(fout, fin, Us, masses, R, 0, cb, 3)
+                assert ms > 0, (R, cb)
+        for kw in ({"cb": 2}, {"iters": 0}, {"rhs_per_thread": 3}):
+            with pytest.raises(w.WilsonError) as e:
+                ctx.time_sloppy(fout, fin, Us, masses, **kw)
+            assert e.value.code == w.WD_EARG
