@@ -191,7 +191,8 @@ typedef struct wd_mixed_params {
     double tol;         /* stop when the FP64 |M^dag b - M^dag M x| < tol |M^dag b| */
     double delta;       /* reliable update when the sloppy |r| < delta x |r| at the last one */
     int spinor_prec;    /* WD_F16 or WD_F32 storage of the inner vectors */
-    int rhs_per_thread; /* 1, 2 or 4 systems per thread, WD_RHS_LOOP; 0 = autotune */
+    int rhs_per_thread; /* 1, 2 or 4 systems per thread, WD_RHS_LOOP; 0 = autotune
+                           (among the WD_TUNE_NR candidates) */
     int site_chunk;     /* site blocks (256 sites) per L2 chunk of the multi-system
                            operator; 0 = off, -1 = autotune (with rhs_per_thread) */
     int check_every;    /* iterations between host polls of the convergence flags */
@@ -259,9 +260,12 @@ int wd_time_sloppy(wd_ctx *ctx, wd_field *out, const wd_field *in, const wd_gaug
                    const double *masses, int rhs_per_thread, int site_chunk, int cb, int iters,
                    double *ms_per_launch);
 
-/* Candidates of the autotuner: rhs per thread {1, 2, 4} x site chunk
- * {0, 16, 32, 64}. */
-#define WD_TUNE_NR 3
+/* Candidates of the autotuner: rhs per thread {1, 2, 4, WD_RHS_LOOP} x site
+ * chunk {0, 16, 32, 64}.  WD_RHS_LOOP is timed (chunk 0 only) for F16 spinors
+ * with F16 12-real links from 8 systems on; the other rhs-per-thread values
+ * give bit-identical results, WD_RHS_LOOP differs by its F16 roundings.  The
+ * choice is cached per context. */
+#define WD_TUNE_NR 4
 #define WD_TUNE_NC 4
 
 /* Autotuner: times the sloppy M over f's nrhs systems for every candidate

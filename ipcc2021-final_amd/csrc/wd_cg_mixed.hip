@@ -896,9 +896,15 @@ int poll_flag(wd_ctx *ctx, const int *stop, const char *who)
 // Times HOP_M over `nrhs` systems for every (rhs per thread, site chunk)
 // candidate and keeps the fastest per (nrhs, spinor format, link format,
 // one or both parities).
-// fixed_R > 0 restricts the search to that rhs-per-thread.
-static const int kTuneR[WD_TUNE_NR] = {1, 2, 4};
+// fixed_R > 0 restricts the search to that rhs-per-thread.  Every system of a
+// site in one thread (WD_RHS_LOOP) is a candidate for F16 spinors with F16
+// 12-real links from kLoopTuneMin systems on, where it shares each link load
+// among many systems; its products round the half spinor to F16, so below
+// that the candidates stay the bit-identical work orders.  The choice is
+// cached per context, so the solves of one context repeat bit for bit.
+static const int kTuneR[WD_TUNE_NR] = {1, 2, 4, WD_RHS_LOOP};
 static const int kTuneC[WD_TUNE_NC] = {0, 16, 32, 64};
+constexpr int kLoopTuneMin = 8;
 
 int wd_tune_sloppy(wd_ctx *ctx, const SloppyArgs &a0, int sprec, int gprec, int recon, int cb,
                    int fixed_R, int *best_R, int *best_C, double *ms_out)
@@ -924,6 +930,9 @@ int wd_tune_sloppy(wd_ctx *ctx, const SloppyArgs &a0, int sprec, int gprec, int 
     for (int ir = 0; ir < WD_TUNE_NR && !rc; ++ir) {
         const int R = kTuneR[ir];
         if (fixed_R > 0 && R != fixed_R)
+            continue;
+        if (R == WD_RHS_LOOP && !(wd_sloppy_loop_supported(sprec, gprec, recon) &&
+                                  (fixed_R == WD_RHS_LOOP || a0.nrhs >= kLoopTuneMin)))
             continue;
         const bool one_group = wd_sloppy_groups(a0.nrhs, R) == 1;
         for (int ic = 0; ic < WD_TUNE_NC && !rc; ++ic) {

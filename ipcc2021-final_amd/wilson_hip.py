@@ -37,7 +37,7 @@ OPT_F64_PAIRS = 5
 OPT_LOOP_PREFETCH = 6
 RECON_18, RECON_12 = 18, 12
 # autotuner candidates (include/wilson.h WD_TUNE_NR x WD_TUNE_NC)
-TUNE_R = (1, 2, 4)
+TUNE_R = (1, 2, 4, RHS_LOOP)
 TUNE_CHUNK = (0, 16, 32, 64)
 TUNE_NR, TUNE_NC = len(TUNE_R), len(TUNE_CHUNK)
 

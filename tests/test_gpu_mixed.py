@@ -230,15 +230,23 @@ def test_tune_report(w):
         f = ctx.field(prec=w.F16, nrhs=6, data=bs)
         best, chunk, ms = ctx.tune_report(f, Us)
         # 6 systems: R = 1, 2, 4 all leave more than one group, so every
-        # candidate is timed
-        assert set(ms) == {(R, C) for R in w.TUNE_R for C in w.TUNE_CHUNK}
+        # candidate is timed; F32 links: no loop-kernel candidate
+        assert set(ms) == {(R, C) for R in (1, 2, 4) for C in w.TUNE_CHUNK}
         assert best in w.TUNE_R and chunk in w.TUNE_CHUNK
         assert all(v > 0 for v in ms.values())
         assert ms[(best, chunk)] == min(ms.values())
         f1 = ctx.field(prec=w.F16, nrhs=1, data=bs[:1])
         _, chunk1, ms1 = ctx.tune_report(f1, Us)
         # one system: a single group for every R, chunk candidates not timed
-        assert set(ms1) == {(R, 0) for R in w.TUNE_R} and chunk1 == 0
+        assert set(ms1) == {(R, 0) for R in (1, 2, 4)} and chunk1 == 0
+        # F16 links: the loop kernel is a candidate from 8 systems on (chunk 0)
+        Uh = ctx.gauge(A, prec=w.F16, recon=12)
+        _, _, ms6 = ctx.tune_report(f, Uh)
+        assert (w.RHS_LOOP, 0) not in ms6
+        f8 = ctx.field(prec=w.F16, nrhs=8, data=bs + bs[:2])
+        best8, chunk8, ms8 = ctx.tune_report(f8, Uh)
+        assert (w.RHS_LOOP, 0) in ms8 and (w.RHS_LOOP, 16) not in ms8
+        assert ms8[(best8, chunk8)] == min(ms8.values())
 
 
 def test_sloppy_argument_errors(w):
