@@ -3,7 +3,9 @@
 //
 // Each call uploads the caller's host arrays, runs one stream-ordered kernel
 // sequence in libwilson.so and downloads dest, so the blocking, fully
-// overwriting semantics of the reference hold.  The scalar family (Dslash,
+// overwriting semantics of the reference hold.  The links are taken from a
+// live gauge_resident of U when there is one (no upload), and the device
+// spinor fields are kept per context between calls.  The scalar family (Dslash,
 // Dslashoffd, DslashEE/OO) and the AVX2 family (DslashNew, DslashoffdNew,
 // DslashEEOONew) compute the same operator and map to the same kernels.
 #include <cstring>
@@ -21,10 +23,11 @@ void hop(lattice_fermion &src, lattice_fermion &dest, lattice_gauge &U, bool dag
     if (src.A == dest.A)
         throw std::runtime_error("Dslashoffd: src and dest must not alias");
     wd_ctx *ctx = wdrt::context(src.subgs, src.site_vec);
-    wdrt::Gauge g(ctx, U);
-    wdrt::Field in(ctx, src), out(ctx);
-    wdrt::check(ctx, wd_hop(ctx, out.f, in.f, g.g, dag ? 1 : 0, cb), "wd_hop");
-    out.download(dest);
+    wdrt::GaugeRef g(ctx, U);
+    wd_field *in = wdrt::scratch_field(ctx, 0), *out = wdrt::scratch_field(ctx, 1);
+    wdrt::upload(ctx, in, src);
+    wdrt::check(ctx, wd_hop(ctx, out, in, g.g, dag ? 1 : 0, cb), "wd_hop");
+    wdrt::download(ctx, out, dest);
 }
 
 void full(lattice_fermion &src, lattice_fermion &dest, lattice_gauge &U, double mass, bool dag)
@@ -32,19 +35,21 @@ void full(lattice_fermion &src, lattice_fermion &dest, lattice_gauge &U, double 
     if (src.A == dest.A)
         throw std::runtime_error("Dslash: src and dest must not alias");
     wd_ctx *ctx = wdrt::context(src.subgs, src.site_vec);
-    wdrt::Gauge g(ctx, U);
-    wdrt::Field in(ctx, src), out(ctx);
-    wdrt::check(ctx, wd_apply_M(ctx, out.f, in.f, g.g, mass, dag ? 1 : 0), "wd_apply_M");
-    out.download(dest);
+    wdrt::GaugeRef g(ctx, U);
+    wd_field *in = wdrt::scratch_field(ctx, 0), *out = wdrt::scratch_field(ctx, 1);
+    wdrt::upload(ctx, in, src);
+    wdrt::check(ctx, wd_apply_M(ctx, out, in, g.g, mass, dag ? 1 : 0), "wd_apply_M");
+    wdrt::download(ctx, out, dest);
 }
 
 // dest = (4+m) src over the full lattice (half = -1) or one half (0 even, 1 odd)
 void diag(lattice_fermion &src, lattice_fermion &dest, double mass, int half)
 {
     wd_ctx *ctx = wdrt::context(src.subgs, src.site_vec);
-    wdrt::Field in(ctx, src), out(ctx);
-    wdrt::check(ctx, wd_diag(ctx, out.f, in.f, mass), "wd_diag");
-    out.download(dest);
+    wd_field *in = wdrt::scratch_field(ctx, 0), *out = wdrt::scratch_field(ctx, 1);
+    wdrt::upload(ctx, in, src);
+    wdrt::check(ctx, wd_diag(ctx, out, in, mass), "wd_diag");
+    wdrt::download(ctx, out, dest);
     if (half >= 0) {
         const int nh = dest.size / 2;
         std::memset(static_cast<void *>(dest.A + (1 - half) * nh), 0,

@@ -32,7 +32,7 @@ int CGinvert(lattice_fermion &src, lattice_fermion &dest, lattice_gauge &U, cons
     for (int i = 0; i < dest.size; i++)
         dest.A[i] = 1.0 * rand() / RAND_MAX;
     wd_ctx *ctx = wdrt::context(src.subgs, src.site_vec);
-    wdrt::Gauge g(ctx, U);
+    wdrt::GaugeRef g(ctx, U);
     wdrt::Field b(ctx, src), x(ctx, dest);
     wd_cg_params p;
     wd_cg_params_default(&p);

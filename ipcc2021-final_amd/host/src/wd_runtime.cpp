@@ -13,6 +13,9 @@ namespace wdrt {
 namespace {
 std::mutex g_mu;
 std::map<std::tuple<int, int, int, int>, wd_ctx *> g_ctx;
+std::map<std::pair<wd_ctx *, int>, wd_field *> g_scratch;
+// resident gauges: lattice_gauge object -> its device copy
+std::map<const lattice_gauge *, Gauge *> g_resident;
 } // namespace
 
 void check(wd_ctx *ctx, int rc, const char *what)
@@ -93,4 +96,84 @@ Gauge::~Gauge()
     wd_gauge_destroy(g);
 }
 
+GaugeRef::GaugeRef(wd_ctx *c, const lattice_gauge &U)
+{
+    {
+        std::lock_guard<std::mutex> lock(g_mu);
+        auto it = g_resident.find(&U);
+        if (it != g_resident.end() && it->second->ctx == c) {
+            g = it->second->g;
+            return;
+        }
+    }
+    own = new Gauge(c, U);
+    g = own->g;
+}
+
+GaugeRef::~GaugeRef()
+{
+    delete own;
+}
+
+void keep_gauge(const lattice_gauge &U)
+{
+    Gauge *copy = new Gauge(context(U.subgs, U.site_vec), U);
+    std::lock_guard<std::mutex> lock(g_mu);
+    auto it = g_resident.find(&U);
+    if (it != g_resident.end()) {
+        delete it->second;
+        it->second = copy;
+    } else {
+        g_resident[&U] = copy;
+    }
+}
+
+void release_gauge(const lattice_gauge &U)
+{
+    Gauge *copy = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_mu);
+        auto it = g_resident.find(&U);
+        if (it == g_resident.end())
+            return;
+        copy = it->second;
+        g_resident.erase(it);
+    }
+    delete copy;
+}
+
+wd_field *scratch_field(wd_ctx *ctx, int slot)
+{
+    std::lock_guard<std::mutex> lock(g_mu);
+    wd_field *&f = g_scratch[std::make_pair(ctx, slot)];
+    if (!f)
+        check(ctx, wd_field_create(ctx, &f, WD_F64, 1), "wd_field_create");
+    return f;
+}
+
+void upload(wd_ctx *ctx, wd_field *f, const lattice_fermion &src)
+{
+    check(ctx, wd_field_upload_ref(ctx, f, reinterpret_cast<const double *>(src.A), 0),
+          "wd_field_upload_ref");
+}
+
+void download(wd_ctx *ctx, const wd_field *f, lattice_fermion &dst)
+{
+    check(ctx, wd_field_download_ref(ctx, f, reinterpret_cast<double *>(dst.A), 0),
+          "wd_field_download_ref");
+}
+
 } // namespace wdrt
+
+// ---- gauge_resident (host/include/gauge_resident.h) ---------------------------
+#include "gauge_resident.h"
+
+gauge_resident::gauge_resident(lattice_gauge &U) : U_(U)
+{
+    wdrt::keep_gauge(U_);
+}
+
+gauge_resident::~gauge_resident()
+{
+    wdrt::release_gauge(U_);
+}

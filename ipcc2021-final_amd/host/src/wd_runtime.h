@@ -37,4 +37,30 @@ struct Gauge {
     Gauge &operator=(const Gauge &) = delete;
 };
 
+// The device links for one operator call: the copy a live gauge_resident
+// keeps for U, else a fresh upload released at the end of the call.
+struct GaugeRef {
+    wd_gauge *g = nullptr;
+    GaugeRef(wd_ctx *c, const lattice_gauge &U);
+    GaugeRef(const GaugeRef &) = delete;
+    GaugeRef &operator=(const GaugeRef &) = delete;
+
+private:
+    Gauge *own = nullptr;
+
+public:
+    ~GaugeRef();
+};
+
+// Two F64 single-system device fields per context, kept between operator
+// calls (a fresh allocation per call cost the drop-in API more than the
+// kernel); slot 0 and slot 1 are distinct.  Valid until the context ends.
+wd_field *scratch_field(wd_ctx *ctx, int slot);
+void upload(wd_ctx *ctx, wd_field *f, const lattice_fermion &src);
+void download(wd_ctx *ctx, const wd_field *f, lattice_fermion &dst);
+
+// gauge_resident's registry (keyed by the lattice_gauge object)
+void keep_gauge(const lattice_gauge &U);
+void release_gauge(const lattice_gauge &U);
+
 } // namespace wdrt

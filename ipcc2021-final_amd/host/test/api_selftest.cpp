@@ -5,14 +5,20 @@
 // usage: api_selftest DIR Lx Ly Lz Lt mass
 //   reads  DIR/src.bin (complex<double>[V*12]) and DIR/gauge.bin (4 x complex<double>[V*9])
 //   writes DIR/<name>.bin for each entry point listed below
+// usage: api_selftest --time Lx Ly Lz Lt reps
+//   times DslashNew on seeded inputs, uploading the links on every call and
+//   with a gauge_resident guard, and prints both averages
 #include <complex>
 #include <cstdio>
 #include <cstdlib>
 #include <string>
 #include <vector>
 
+#include <chrono>
+
 #include "dslash.h"
 #include "dslash_new.h"
+#include "gauge_resident.h"
 #include "invert.h"
 
 typedef std::complex<double> cd;
@@ -37,8 +43,48 @@ static void write_all(const std::string &fn, const void *p, size_t bytes)
     std::fclose(f);
 }
 
+static int time_mode(char **argv)
+{
+    int L[4], S[4];
+    long V = 1;
+    for (int i = 0; i < 4; ++i) {
+        L[i] = S[i] = std::atoi(argv[2 + i]);
+        V *= L[i];
+    }
+    const int reps = std::atoi(argv[6]);
+    lattice_fermion src(S, L), out(S, L);
+    lattice_gauge U(S, L);
+    srand(5);
+    for (long k = 0; k < 12 * V; ++k)
+        src.A[k] = cd((double)rand() / RAND_MAX, (double)rand() / RAND_MAX);
+    for (int mu = 0; mu < 4; ++mu)
+        for (long k = 0; k < 9 * V; ++k)
+            U.A[mu][k] = cd((double)rand() / RAND_MAX - 0.5, (double)rand() / RAND_MAX - 0.5);
+    auto run = [&]() {
+        DslashNew(src, out, U, 0.1, false); // first call: context, fields
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int k = 0; k < reps; ++k)
+            DslashNew(src, out, U, 0.1, false);
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                   .count() /
+               reps;
+    };
+    const double up = run();
+    double kept;
+    {
+        gauge_resident keep(U);
+        kept = run();
+    }
+    std::printf("api_selftest: DslashNew %dx%dx%dx%d, links uploaded per call %.3f ms, "
+                "gauge_resident %.3f ms\n",
+                L[0], L[1], L[2], L[3], up, kept);
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
+    if (argc == 7 && std::string(argv[1]) == "--time")
+        return time_mode(argv);
     if (argc != 7) {
         std::fprintf(stderr, "usage: %s DIR Lx Ly Lz Lt mass\n", argv[0]);
         return 1;
@@ -85,11 +131,28 @@ int main(int argc, char **argv)
     write_all(dir + "/diag_ee.bin", out.A, bytes);
     DslashOO(src, out, mass);
     write_all(dir + "/diag_oo.bin", out.A, bytes);
+    // the same operators with the links kept on the device by a guard
+    {
+        gauge_resident keep(U);
+        for (int dag = 0; dag < 2; ++dag) {
+            DslashNew(src, out, U, mass, dag != 0);
+            write_all(dir + "/Mnew_res_d" + std::to_string(dag) + ".bin", out.A, bytes);
+            DslashoffdNew(src, out, U, dag != 0, 1);
+            write_all(dir + "/hopnew_res_d" + std::to_string(dag) + ".bin", out.A, bytes);
+        }
+    }
     // CGinvert on the same system, start vector from srand(11)
     srand(11);
     lattice_fermion x(S, L);
     CGinvert(src, x, U, mass, 1000, 1e-9);
     write_all(dir + "/cg_x.bin", x.A, bytes);
+    // after the guard is gone the links are read from U again: all-zero links
+    // make M = (4 + m) 1
+    for (int mu = 0; mu < 4; ++mu)
+        for (long k = 0; k < 9 * V; ++k)
+            U.A[mu][k] = 0.0;
+    DslashNew(src, out, U, mass, false);
+    write_all(dir + "/Mnew_zero.bin", out.A, bytes);
     std::printf("api_selftest: done\n");
     return 0;
 }

@@ -47,6 +47,12 @@ def test_cpp_api_matches_reference(name, tmp_path):
     ee, oo = out("diag_ee").reshape(2, -1), out("diag_oo").reshape(2, -1)
     assert rel_l2(ee[0], diag[0]) < 1e-15 and np.all(ee[1] == 0)
     assert rel_l2(oo[1], diag[1]) < 1e-15 and np.all(oo[0] == 0)
+    # links kept on the device by gauge_resident: the same outputs bit for bit
+    for dag in (0, 1):
+        assert np.array_equal(out(f"Mnew_res_d{dag}"), out(f"Mnew_d{dag}"))
+        assert np.array_equal(out(f"hopnew_res_d{dag}"), out(f"hopnew_d{dag}_cb1"))
+    # after the guard, changed links are read again (zero links: M = 4 + m)
+    assert rel_l2(out("Mnew_zero"), (4.0 + mass) * f["src"]) < 1e-15
     # CGinvert: x0 from srand(11), |r| < 1e-9 absolute; check with the FP64 oracle
     o = Oracle()
     A = [np.ascontiguousarray(a) for a in f["gauge"]]
@@ -56,3 +62,15 @@ def test_cpp_api_matches_reference(name, tmp_path):
     assert np.linalg.norm(r) < 1.5e-9
     it, xo, _ = o.cg(b, libc_rand_x0(11, V * 12), A, L, mass, 1000, 1e-9)
     assert it > 0 and rel_l2(x, xo) < 1e-8
+
+
+def test_gauge_resident_removes_the_link_upload():
+    """DslashNew on the drop-in API at 32^3x64: with a gauge_resident guard the
+    1.2 GB of links are not uploaded on every call."""
+    p = subprocess.run([EXE, "--time", "32", "32", "32", "64", "5"], capture_output=True,
+                       text=True, timeout=300)
+    assert p.returncode == 0, p.stderr
+    print(p.stdout.strip(), flush=True)
+    m = [float(v) for v in
+         __import__("re").findall(r"([0-9.]+) ms", p.stdout)]
+    assert len(m) == 2 and m[1] < m[0]
