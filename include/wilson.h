@@ -340,11 +340,23 @@ int wd_time_op(wd_ctx *ctx, int op, wd_field *out, const wd_field *in, const wd_
  * once for both (links are three quarters of an FP64 hop's bytes); 0 = one
  * system per thread.  Results are bit-identical. */
 #define WD_OPT_F64_PAIRS 5
-/* The WD_RHS_LOOP sloppy kernel's neighbour prefetch: 3 (default) = the
- * loads of the third next hop direction are issued before each direction's
- * arithmetic, at 2 wavefronts per SIMD; 1 = the next direction's, at 3
- * wavefronts per SIMD.  Results are bit-identical. */
+/* The WD_RHS_LOOP sloppy kernel's neighbour prefetch: 1 = the loads of the
+ * next hop direction are issued before each direction's arithmetic, at 3
+ * wavefronts per SIMD; 3 = the third next direction's, at 2 wavefronts per
+ * SIMD; 0 (default) = 1 for the hop and M launches, 3 for the launches fused
+ * with dot products (which would spill registers at 3 wavefronts per SIMD).
+ * Results are bit-identical. */
 #define WD_OPT_LOOP_PREFETCH 6
+/* Systems per workgroup of the WD_RHS_LOOP sloppy kernel: the systems run in
+ * groups, each group a separate run of workgroups over all sites (group 0 for
+ * every site block, then group 1, ...), so neighbouring workgroups reach the
+ * same system at nearly the same time and share its spinors through L2, at
+ * the cost of loading the links once per group.  Groups are formed from the
+ * systems a launch applies (a solver's masked systems are left out).  0
+ * (default) = automatic: balanced groups of at most 12 systems; S > 0 =
+ * balanced groups of at most S (S >= the system count: one group).  Results
+ * are bit-identical. */
+#define WD_OPT_LOOP_SYSTEMS 7
 int wd_set_option(wd_ctx *ctx, int option, int value);
 /* The value an option has in effect on this context: 0/1 for
  * WD_OPT_XCD_REMAP, WD_OPT_PARITY_ORDER and WD_OPT_F64_PAIRS, the column width W (0 = natural

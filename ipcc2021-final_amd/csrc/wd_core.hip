@@ -124,7 +124,8 @@ extern "C" int wd_create(wd_ctx **out, const int L[4], const int bc[4], int devi
     c->parity_il = 1;
     c->ts_tb = wd_auto_tstream(c);
     c->f64_pairs = 1;
-    c->loop_prefetch = 3;
+    c->loop_prefetch = 0;
+    c->loop_systems = 0;
     // partials: enough for a both-parity hop launch and the streaming
     // reduction grids (kStreamPartials, 256 KiB)
     const int hop_blocks = (int)(2 * ((Vh + kBlock - 1) / kBlock));
@@ -620,6 +621,9 @@ extern "C" int wd_get_option(wd_ctx *ctx, int option, int *value)
     case WD_OPT_LOOP_PREFETCH:
         *value = ctx->loop_prefetch;
         return WD_OK;
+    case WD_OPT_LOOP_SYSTEMS:
+        *value = ctx->loop_systems;
+        return WD_OK;
     default:
         return wd_set_error(ctx, WD_EARG, "get_option: unknown option");
     }
@@ -646,8 +650,13 @@ extern "C" int wd_set_option(wd_ctx *ctx, int option, int value)
         ctx->f64_pairs = value ? 1 : 0;
         return WD_OK;
     case WD_OPT_LOOP_PREFETCH:
-        WD_ARG_CHECK(ctx, value == 1 || value == 3, "set_option: loop prefetch must be 1 or 3");
+        WD_ARG_CHECK(ctx, value == 0 || value == 1 || value == 3,
+                     "set_option: loop prefetch must be 0, 1 or 3");
         ctx->loop_prefetch = value;
+        return WD_OK;
+    case WD_OPT_LOOP_SYSTEMS:
+        WD_ARG_CHECK(ctx, value >= 0, "set_option: systems per workgroup must be >= 0");
+        ctx->loop_systems = value;
         return WD_OK;
     case WD_OPT_BLOCK_TILE:
         WD_ARG_CHECK(ctx, value >= 0, "set_option: block tile width must be >= 0");

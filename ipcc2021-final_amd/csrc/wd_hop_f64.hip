@@ -504,7 +504,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 1))) 
         const int ib = ((tb * g.Lz + z) * g.Ly + y) * g.Lxh + xh;
         cpx<double> psi[4][3], W[3][3], h[2][3];
         load_spinor(in + (long)pb * 12 * stride, stride, ib, psi);
-        load_link<false>(U + (long)((3 * 2 + pb) * 9) * stride, stride, ib, W);
+        load_link<true>(U + (long)((3 * 2 + pb) * 9) * stride, stride, ib, W);
         using D = Dir<3, false, DAG>;
         project<D>(psi, h);
         su3_mul_half<true>(W, h, rc);
@@ -531,9 +531,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 1))) 
         hop_mu<0, DAG, false>(acc, in_o, U, g, p, i, row + (xf >> 1), row + (xb >> 1), sxf, sxb);
         hop_mu<1, DAG, false>(acc, in_o, U, g, p, i, i + dyf, i + dyb, syf, syb);
         hop_mu<2, DAG, false>(acc, in_o, U, g, p, i, i + dzf, i + dzb, szf, szb);
-        // forward t with the site's t link, kept for the next site's backward hop
+        // forward t with the site's t link, kept for the next site's backward
+        // hop; no other thread reads it, so it loads non-temporal and leaves
+        // L2 to the spinors and x, y, z links the neighbouring columns share
+        // (32^3x64 M 0.447 -> 0.437 ms on MI355X)
         cpx<double> W[3][3];
-        load_link<false>(U + (long)((3 * 2 + p) * 9) * stride, stride, i, W);
+        load_link<true>(U + (long)((3 * 2 + p) * 9) * stride, stride, i, W);
         {
             const int jf = (t + 1 == g.Lt) ? i - (g.Lt - 1) * slice : i + slice;
             cpx<double> psi[4][3], h[2][3], r[2][3];

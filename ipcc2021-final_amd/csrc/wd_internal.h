@@ -57,7 +57,8 @@ struct wd_ctx {
     int parity_il;        // 1: both-parity FP64 launches alternate the parities per site block
     int ts_tb;            // t-run length of the t-streaming both-parity FP64 kernel, 0 = off
     int f64_pairs;        // 1: batched FP64 hops of several systems run two systems per thread
-    int loop_prefetch;    // WD_RHS_LOOP kernel: neighbour loads 1 (3 wavefronts/SIMD) or 3 (2) ahead
+    int loop_prefetch;    // WD_RHS_LOOP kernel: neighbour loads 1 (3 wavefronts/SIMD) or 3 (2) ahead, 0 = by mode
+    int loop_systems;     // WD_RHS_LOOP kernel: systems per workgroup (0 = all)
     // autotuned (rhs per thread, chunk) of the sloppy operator, keyed by
     // (nrhs, spinor precision, 100 * gauge precision + recon)
     std::map<std::tuple<int, int, int>, std::pair<int, int>> tuned;

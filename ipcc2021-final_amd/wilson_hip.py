@@ -35,6 +35,7 @@ OPT_PARITY_ORDER = 3
 OPT_TSTREAM = 4
 OPT_F64_PAIRS = 5
 OPT_LOOP_PREFETCH = 6
+OPT_LOOP_SYSTEMS = 7
 RECON_18, RECON_12 = 18, 12
 # autotuner candidates (include/wilson.h WD_TUNE_NR x WD_TUNE_NC)
 TUNE_R = (1, 2, 4, RHS_LOOP)

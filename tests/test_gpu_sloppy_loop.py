@@ -130,3 +130,45 @@ def test_time_sloppy(w):
             with pytest.raises(w.WilsonError) as e:
                 ctx.time_sloppy(fout, fin, Us, masses, **kw)
             assert e.value.code == w.WD_EARG
+
+
+def test_loop_groups_and_prefetch_bit_identical(w):
+    """WD_OPT_LOOP_SYSTEMS (systems per workgroup group) and
+    WD_OPT_LOOP_PREFETCH only reorder work: M, M^dagger and a mixed Schur
+    solve give the same bits for every group size and both prefetch depths,
+    including groups that do not divide the system count."""
+    L = [6, 4, 6, 10]
+    n = 11
+    A, bs = _setup(w, L, n)
+    masses = [kappa_to_mass(0.11 + 0.002 * r) for r in range(n)]
+    with w.Context(L, bc=[1, 1, 1, -1]) as ctx:
+        assert ctx.get_option(w.OPT_LOOP_SYSTEMS) == 0
+        assert ctx.get_option(w.OPT_LOOP_PREFETCH) == 0
+        U = ctx.gauge(A)
+        Us = ctx.gauge(A, prec=w.F16, recon=12)
+        fb = ctx.field(nrhs=n, data=bs)
+        fin = ctx.field(prec=w.F16, nrhs=n, data=bs)
+        fout = ctx.field(prec=w.F16, nrhs=n)
+        runs = {}
+        for K in (0, 1, 3):
+            for per in (0, 1, 4, 5, 11, 64):
+                ctx.set_option(w.OPT_LOOP_PREFETCH, K)
+                ctx.set_option(w.OPT_LOOP_SYSTEMS, per)
+                assert ctx.get_option(w.OPT_LOOP_SYSTEMS) == per
+                got = []
+                for dag in (False, True):
+                    ctx.apply_M_sloppy(fout, fin, Us, masses, dag, w.RHS_LOOP)
+                    got += [fout.download(r) for r in range(n)]
+                x = ctx.field(nrhs=n)
+                res = ctx.cg_mixed(x, fb, U, Us, masses, tol=1e-10, rhs_per_thread=w.RHS_LOOP,
+                                   even_odd=True)
+                assert res["status"] == 0
+                got += [x.download(r) for r in range(n)]
+                runs[(K, per)] = (got, [s["iterations"] for s in res["systems"]])
+        with pytest.raises(w.WilsonError) as e:
+            ctx.set_option(w.OPT_LOOP_SYSTEMS, -1)
+        assert e.value.code == w.WD_EARG
+    base_got, base_its = runs[(0, 0)]
+    for key, (got, its) in runs.items():
+        assert its == base_its, key
+        assert all(np.array_equal(a, b) for a, b in zip(got, base_got)), key
