@@ -537,6 +537,39 @@ def batched_case(w):
     }
 
 
+def link_product_case(w, ctx, nsys, products_per_M, ms_M):
+    """MFMA against VALU for the operator's link product, U h for one link and
+    the half spinors of nsys systems (wd_link_product): 2^18 sites, 64
+    repetitions on registers per launch (compute-bound), the same F16 inputs
+    for both kernels.  144 flop per product (2 spins x 3 x 3 complex
+    multiply-adds).  `product_ms_per_M` scales each rate to the products of
+    one loop-kernel M launch (8 per site and system) and `share_of_M` divides
+    by the measured launch time: what the product would cost if it ran
+    alone at that rate."""
+    nsites, inner = 1 << 18, 64
+    rng = np.random.default_rng(20211028)
+    U = rng.standard_normal((nsites, 3, 3)) + 1j * rng.standard_normal((nsites, 3, 3))
+    h = rng.standard_normal((nsites, nsys, 2, 3)) + 1j * rng.standard_normal((nsites, nsys, 2, 3))
+    out, outs = {}, {}
+    for name, impl in (("valu", w.LINKPROD_VALU), ("mfma", w.LINKPROD_MFMA)):
+        o, ms = ctx.link_product(impl, U, h, inner=inner, iters=5)
+        prods = nsites * nsys * inner
+        ps = ms * 1e9 / prods
+        out[name] = {"ms": round(ms, 4),
+                     "tflops": round(144 * prods / (ms * 1e-3) / 1e12, 2),
+                     "ps_per_product": round(ps, 3),
+                     "product_ms_per_M": round(ps * products_per_M * 1e-9, 4),
+                     "share_of_M": round(ps * products_per_M * 1e-9 / ms_M, 4)}
+        outs[name] = o
+    out["mfma_speedup_on_the_product"] = round(out["valu"]["ms"] / out["mfma"]["ms"], 3)
+    out["max_rel_diff_mfma_vs_valu"] = float(np.abs(outs["mfma"] - outs["valu"]).max()
+                                            / np.abs(outs["valu"]).max())
+    out["workload"] = (f"{nsites} sites x {nsys} systems, {inner} repetitions per launch; "
+                       "VALU: one site per lane, v_dot2_f32_f16; MFMA: v_mfma_f32_16x16x16f16, "
+                       "two sites per tile")
+    return out
+
+
 def large_case(w, full_system=False):
     """48^3x96, 12 Gaussian systems at kappa 0.135, F16 spinors and F16 12-real
     links; the mixed CG on the full M^dag M (about 13 s) only with full_system."""
@@ -590,6 +623,8 @@ def large_case(w, full_system=False):
         fout = ctx.field(prec=w.F16, nrhs=n)
         ms_loop = ctx.time_sloppy(fout, fin, Us, [mass] * n, w.RHS_LOOP, 0, -1, 5)
         del fout
+        log("large: MFMA vs VALU link product")
+        linkprod = link_product_case(w, ctx, n, 8 * w.volume(L) * n, ms_loop)
     V = w.volume(L)
     bytes_site = 8 * 24 / n + 3 * 52   # F16 12-real links shared by n systems, F16 spinors
     cg_systems = mxe["systems"] + (mx["systems"] if mx else [])
@@ -619,6 +654,7 @@ def large_case(w, full_system=False):
         "sloppy_M_GBps_best": round(bytes_site * V * n / (ms[(bestR, bestC)] * 1e-3) / 1e9, 1),
         "sloppy_M_ms_loop_kernel": round(ms_loop, 3),
         "sloppy_M_GBps_loop_kernel": round(bytes_site * V * n / (ms_loop * 1e-3) / 1e9, 1),
+        "mfma_vs_valu": linkprod,
     }
     if mx:
         out.update({"mixed_seconds": round(mx["seconds"], 3),

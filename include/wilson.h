@@ -27,6 +27,7 @@
 #define WILSON_H
 
 #include <stddef.h>
+#include <stdint.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -311,6 +312,23 @@ int wd_pion_correlator_noalloc(wd_ctx *ctx, const wd_field *S, int rhs_first, in
 int wd_time_op(wd_ctx *ctx, int op, wd_field *out, const wd_field *in, const wd_gauge *U,
                double mass, int dagger, int cb, int iters, double *ms_per_launch);
 
+/* MFMA-vs-VALU experiment for the multi-system operator's link product
+ * ("3x3 by 3x2N": one F16 SU(3) link times the F16 half spinors of N systems at
+ * its site, FP32 accumulation).  No counterpart in the reference (AVX2 CPU
+ * code).  Inputs are host arrays: U [nsites][3][3] and h [nsites][nsys][2][3]
+ * of packed (re, im) F16 pairs (real part in the low 16 bits); out [nsites]
+ * [nsys][2][3] (re, im) floats receives the products summed over `inner`
+ * repetitions (inner x U h up to FP32 rounding).  The kernel runs once
+ * untimed and `iters` times timed (HIP events on the context stream);
+ * ms_per_launch is the average.  WD_LINKPROD_VALU: one site per lane, packed
+ * F16 dot products (the loop sloppy kernel's arithmetic); WD_LINKPROD_MFMA:
+ * v_mfma_f32_16x16x16f16 with two sites' links block-diagonal in one tile and
+ * 8 systems x 2 spins as its columns.  nsites even, nsys in 1 .. 64. */
+#define WD_LINKPROD_VALU 0
+#define WD_LINKPROD_MFMA 1
+int wd_link_product(wd_ctx *ctx, int impl, int nsites, int nsys, const uint32_t *U,
+                    const uint32_t *h, int inner, int iters, float *out, double *ms_per_launch);
+
 #define WD_OPT_XCD_REMAP 1 /* 1 (default): each XCD processes a contiguous site range */
 /* Work order of the Dslash kernels: 0 = natural site order; W > 0 = z-t tiled
  * order with columns of W site blocks (256 sites each), so the t neighbours of
@@ -320,11 +338,13 @@ int wd_time_op(wd_ctx *ctx, int op, wd_field *out, const wd_field *in, const wd_
  * Default: columns of 4 z-planes, W = round(4 * L0/2 * L1 / 256).  Every W
  * gives bit-identical results. */
 #define WD_OPT_BLOCK_TILE 2
-/* Parity order of the both-parity FP64 launches (M, M^dagger): 0 = all blocks
- * of parity 0, then parity 1; 1 (default) = the two parities of each site
- * block side by side, sharing each link's forward and backward use through
- * L2 (links then also load through L2 in the both-parity sloppy operator).
- * Results are bit-identical. */
+/* Parity order of the both-parity FP64 launches (M, M^dagger) and of the
+ * WD_RHS_LOOP sloppy kernel: 0 = all blocks of parity 0, then parity 1; 1
+ * (default) = the two parities of each site block side by side, sharing each
+ * link's forward and backward use through L2.  The sloppy kernel with R
+ * systems per thread always runs the parities side by side; this option
+ * makes its links load through L2 (1) or non-temporal (0).  Results are
+ * bit-identical. */
 #define WD_OPT_PARITY_ORDER 3
 /* t-streaming of the both-parity FP64 launches (M, M^dagger and the CG's
  * M^dagger with its dot product, one system): each thread owns a column

@@ -250,11 +250,12 @@ struct LoopShape<3> {
 // One site block (kBlock sites of one parity) for systems [r0, r1).
 template <bool DAG, int MODE, int K>
 __device__ __forceinline__ void loop_block(const SloppyArgs &a, const Geom &g, int sb, int cb_first,
-                                           int nparity, int r0, int r1)
+                                           int nparity, int il, int r0, int r1)
 {
     // both parities alternate over the same site range (shared L2 reuse)
-    const int p = (nparity == 2) ? (sb & 1) : cb_first;
-    const int blk = block_order((nparity == 2) ? (sb >> 1) : sb, g);
+    // both parities: side by side (il) or parity 0 first, then parity 1
+    const int p = (nparity == 2) ? (il ? (sb & 1) : sb / a.bpp) : cb_first;
+    const int blk = block_order((nparity == 2) ? (il ? (sb >> 1) : sb - p * a.bpp) : sb, g);
     const int nat = (nparity == 2) ? 2 * blk + p : blk; // natural index (partials)
     const int i = blk * kBlock + threadIdx.x;
     const int stride = g.stride;
@@ -419,7 +420,7 @@ __device__ __forceinline__ int nth_set_bit(unsigned long long m, int j)
 
 template <bool DAG, int MODE, int K>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(LoopShape<K>::waves))) void
-k_hop_sloppy_loop(SloppyArgs a, Geom g, int cb_first, int nparity, int remap, int per)
+k_hop_sloppy_loop(SloppyArgs a, Geom g, int cb_first, int nparity, int remap, int per, int il)
 {
     if (a.stop && *a.stop)
         return;
@@ -461,7 +462,7 @@ k_hop_sloppy_loop(SloppyArgs a, Geom g, int cb_first, int nparity, int remap, in
             k += c;
         }
     }
-    loop_block<DAG, MODE, K>(a, g, sb, cb_first, nparity, r0, r1);
+    loop_block<DAG, MODE, K>(a, g, sb, cb_first, nparity, il, r0, r1);
 }
 
 template <bool DAG, int K>
@@ -471,21 +472,21 @@ void launch_mode(wd_ctx *ctx, const SloppyArgs &a, const Geom &g, int cb_first, 
     switch (mode) {
     case HOP_ONLY:
         hipLaunchKernelGGL((k_hop_sloppy_loop<DAG, HOP_ONLY, K>), dim3(grid), dim3(kBlock), 0,
-                           ctx->stream, a, g, cb_first, nparity, ctx->xcd_remap, per);
+                           ctx->stream, a, g, cb_first, nparity, ctx->xcd_remap, per, ctx->parity_il);
         break;
     case HOP_M:
         hipLaunchKernelGGL((k_hop_sloppy_loop<DAG, HOP_M, K>), dim3(grid), dim3(kBlock), 0,
-                           ctx->stream, a, g, cb_first, nparity, ctx->xcd_remap, per);
+                           ctx->stream, a, g, cb_first, nparity, ctx->xcd_remap, per, ctx->parity_il);
         break;
     case HOP_M_DOT_NORM:
         if constexpr (!DAG)
             hipLaunchKernelGGL((k_hop_sloppy_loop<false, HOP_M_DOT_NORM, K>), dim3(grid),
                                dim3(kBlock), 0, ctx->stream, a, g, cb_first, nparity,
-                               ctx->xcd_remap, per);
+                               ctx->xcd_remap, per, ctx->parity_il);
         break;
     default:
         hipLaunchKernelGGL((k_hop_sloppy_loop<DAG, HOP_M_DOT, K>), dim3(grid), dim3(kBlock), 0,
-                           ctx->stream, a, g, cb_first, nparity, ctx->xcd_remap, per);
+                           ctx->stream, a, g, cb_first, nparity, ctx->xcd_remap, per, ctx->parity_il);
         break;
     }
 }

@@ -36,6 +36,7 @@ OPT_TSTREAM = 4
 OPT_F64_PAIRS = 5
 OPT_LOOP_PREFETCH = 6
 OPT_LOOP_SYSTEMS = 7
+LINKPROD_VALU, LINKPROD_MFMA = 0, 1
 RECON_18, RECON_12 = 18, 12
 # autotuner candidates (include/wilson.h WD_TUNE_NR x WD_TUNE_NC)
 TUNE_R = (1, 2, 4, RHS_LOOP)
@@ -54,7 +55,7 @@ EXPORTED = [
     "wd_cg_params_default", "wd_cg", "wd_cg_eo", "wd_time_op", "wd_set_option",
     "wd_get_option",
     "wd_mixed_params_default", "wd_cg_mixed", "wd_bicgstab_mixed", "wd_apply_M_sloppy",
-    "wd_tune_report", "wd_time_sloppy",
+    "wd_tune_report", "wd_time_sloppy", "wd_link_product",
     "wd_pion_correlator", "wd_pion_correlator_noalloc",
     "wdg_gauge_random_su3", "wdg_gauge_near_identity", "wdg_gauge_unit",
     "wdg_gauge_flip_boundary", "wdg_spinor_gaussian", "wdg_spinor_point", "wdg_philox4x32",
@@ -166,6 +167,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "wd_apply_M_sloppy": ([vp, vp, vp, vp, dp, i, i, i], i),
         "wd_tune_report": ([vp, vp, vp, ctypes.POINTER(i), ctypes.POINTER(i), dp], i),
         "wd_time_sloppy": ([vp, vp, vp, vp, dp, i, i, i, i, ctypes.POINTER(d)], i),
+        "wd_link_product": ([vp, i, i, i, vp, vp, i, i, vp, ctypes.POINTER(d)], i),
         "wd_time_op": ([vp, i, vp, vp, vp, d, i, i, i, ctypes.POINTER(d)], i),
         "wd_pion_correlator": ([vp, vp, i, i, i, dp], i),
         "wd_pion_correlator_noalloc": ([vp, vp, i, i, i, dp], i),
@@ -357,6 +359,27 @@ class Context:
                                            site_chunk, cb, iters, ctypes.byref(ms)),
                    "wd_time_sloppy")
         return ms.value
+
+    def link_product(self, impl: int, U: np.ndarray, h: np.ndarray, inner: int = 1,
+                     iters: int = 1):
+        """MFMA-vs-VALU link-product experiment (wd_link_product).  U: complex
+        [nsites, 3, 3], h: complex [nsites, nsys, 2, 3] (rounded to F16).
+        Returns (U h summed over `inner` repetitions as complex64 [nsites,
+        nsys, 2, 3], average launch ms)."""
+        nsites, nsys = h.shape[0], h.shape[1]
+        assert U.shape == (nsites, 3, 3) and h.shape == (nsites, nsys, 2, 3)
+
+        def pack(z):
+            re = np.ascontiguousarray(z.real, dtype=np.float16).view(np.uint16).astype(np.uint32)
+            im = np.ascontiguousarray(z.imag, dtype=np.float16).view(np.uint16).astype(np.uint32)
+            return np.ascontiguousarray(re | (im << 16))
+        Up, hp = pack(U), pack(h)
+        out = np.empty((nsites, nsys, 2, 3, 2), dtype=np.float32)
+        ms = ctypes.c_double()
+        self.check(self.lib.wd_link_product(self.h, impl, nsites, nsys, _ptr(Up), _ptr(hp), inner,
+                                            iters, _ptr(out), ctypes.byref(ms)),
+                   "wd_link_product")
+        return out[..., 0] + 1j * out[..., 1], ms.value
 
     def tune_report(self, f, Us):
         """Times every (rhs per thread, site chunk) candidate of the sloppy M over
