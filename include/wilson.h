@@ -195,7 +195,9 @@ typedef struct wd_mixed_params {
     int rhs_per_thread; /* 1, 2 or 4 systems per thread, WD_RHS_LOOP; 0 = autotune
                            (among the WD_TUNE_NR candidates) */
     int site_chunk;     /* site blocks (256 sites) per L2 chunk of the multi-system
-                           operator; 0 = off, -1 = autotune (with rhs_per_thread) */
+                           operator; 0 = off, -1 = autotune (with rhs_per_thread).
+                           With WD_RHS_LOOP: systems per pass of the kernel
+                           (0 = WD_OPT_LOOP_SYSTEMS) */
     int check_every;    /* iterations between host polls of the convergence flags */
     int use_graph;      /* replay each block of check_every iterations as one hipGraph */
     int even_odd;       /* 1: iterate on the Schur complement of the even sites (as
@@ -214,7 +216,8 @@ typedef struct wd_mixed_report {
                              (even_odd: at the last check of the full system) */
     double rhs_norm;      /* |M^dag b| */
     int rhs_per_thread;   /* the rhs-per-thread the operator ran with */
-    int site_chunk;       /* the L2 chunk the operator ran with */
+    int site_chunk;       /* the L2 chunk the operator ran with (WD_RHS_LOOP: systems
+                             per pass, 0 = the context's choice) */
 } wd_mixed_report;
 
 /* max_iter 100000, tol 1e-10, delta 2e-2, F16 spinors, autotuned rhs per
@@ -248,7 +251,8 @@ int wd_bicgstab_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd_gaug
 /* out_r = M_r in_r (dagger 0) or M_r^dag in_r with the sloppy operator for all
  * nrhs systems (fields F16 or F32 of the same format, masses[r] host array),
  * rhs_per_thread 1, 2, 4 or WD_RHS_LOOP, site_chunk >= 0 (work order only:
- * every chunk gives bit-identical results). */
+ * every chunk gives bit-identical results; with WD_RHS_LOOP it is the largest
+ * number of systems per pass, 0 = WD_OPT_LOOP_SYSTEMS). */
 int wd_apply_M_sloppy(wd_ctx *ctx, wd_field *out, const wd_field *in, const wd_gauge *Us,
                       const double *masses, int dagger, int rhs_per_thread, int site_chunk);
 
@@ -262,10 +266,12 @@ int wd_time_sloppy(wd_ctx *ctx, wd_field *out, const wd_field *in, const wd_gaug
                    double *ms_per_launch);
 
 /* Candidates of the autotuner: rhs per thread {1, 2, 4, WD_RHS_LOOP} x site
- * chunk {0, 16, 32, 64}.  WD_RHS_LOOP is timed (chunk 0 only) for F16 spinors
- * with F16 12-real links from 8 systems on; the other rhs-per-thread values
- * give bit-identical results, WD_RHS_LOOP differs by its F16 roundings.  The
- * choice is cached per context. */
+ * chunk {0, 16, 32, 64}.  WD_RHS_LOOP is timed for F16 spinors with F16
+ * 12-real links from 8 systems on, its chunk values counting systems per
+ * pass (0 = WD_OPT_LOOP_SYSTEMS, 12 by default; values that give the same
+ * number of passes are timed once); the other rhs-per-thread values give
+ * bit-identical results, WD_RHS_LOOP differs by its F16 roundings (all its
+ * pass sizes give the same bits).  The choice is cached per context. */
 #define WD_TUNE_NR 4
 #define WD_TUNE_NC 4
 

@@ -900,8 +900,11 @@ int poll_flag(wd_ctx *ctx, const int *stop, const char *who)
 // site in one thread (WD_RHS_LOOP) is a candidate for F16 spinors with F16
 // 12-real links from kLoopTuneMin systems on, where it shares each link load
 // among many systems; its products round the half spinor to F16, so below
-// that the candidates stay the bit-identical work orders.  The choice is
-// cached per context, so the solves of one context repeat bit for bit.
+// that the candidates stay the bit-identical work orders.  For WD_RHS_LOOP the
+// chunk candidates are systems per pass (0: the context's choice, 12 by
+// default), so the systems-per-thread axis runs 1, 2, 4 and 12 .. 64; group
+// sizes give bit-identical results.  The choice is cached per context, so the
+// solves of one context repeat bit for bit.
 static const int kTuneR[WD_TUNE_NR] = {1, 2, 4, WD_RHS_LOOP};
 static const int kTuneC[WD_TUNE_NC] = {0, 16, 32, 64};
 constexpr int kLoopTuneMin = 8;
@@ -934,10 +937,21 @@ int wd_tune_sloppy(wd_ctx *ctx, const SloppyArgs &a0, int sprec, int gprec, int 
         if (R == WD_RHS_LOOP && !(wd_sloppy_loop_supported(sprec, gprec, recon) &&
                                   (fixed_R == WD_RHS_LOOP || a0.nrhs >= kLoopTuneMin)))
             continue;
-        const bool one_group = wd_sloppy_groups(a0.nrhs, R) == 1;
+        const bool one_group = R != WD_RHS_LOOP && wd_sloppy_groups(a0.nrhs, R) == 1;
+        int loop_groups_seen = 0; // bit g: a loop candidate with g + 1 groups was timed
         for (int ic = 0; ic < WD_TUNE_NC && !rc; ++ic) {
             if (one_group && ic > 0)
                 break; // the work order cannot change with a single group
+            if (R == WD_RHS_LOOP) {
+                // the chunk candidate is the loop kernel's systems per pass;
+                // candidates with the same number of passes are the same launch
+                const int cap = wd_loop_group_cap(ctx, a0.nrhs, kTuneC[ic]);
+                const int ng = (a0.nrhs + cap - 1) / cap;
+                if (ng <= 31 && (loop_groups_seen >> (ng - 1) & 1))
+                    continue;
+                if (ng <= 31)
+                    loop_groups_seen |= 1 << (ng - 1);
+            }
             SloppyArgs a = a0;
             a.chunk = kTuneC[ic];
             for (int w = 0; w < 2 && !rc; ++w)

@@ -500,6 +500,12 @@ void launch_mode(wd_ctx *ctx, const SloppyArgs &a, const Geom &g, int cb_first, 
 // group (2 groups of 6: CG +3 % / 0 %).
 constexpr int kLoopGroup = 12;
 
+int wd_loop_group_cap(const wd_ctx *ctx, int nrhs, int chunk)
+{
+    const int cap = chunk > 0 ? chunk : ctx->loop_systems > 0 ? ctx->loop_systems : kLoopGroup;
+    return nrhs < 1 ? 1 : min(cap, nrhs);
+}
+
 // Called by wd_launch_hop_sloppy (which has checked the arguments and filled
 // bpp, nsb, tlinks and one) for R = WD_RHS_LOOP.
 int wd_launch_hop_sloppy_loop(wd_ctx *ctx, const SloppyArgs &a, const Geom &g, int dagger,
@@ -509,7 +515,7 @@ int wd_launch_hop_sloppy_loop(wd_ctx *ctx, const SloppyArgs &a, const Geom &g, i
     const int cb_first = (cb < 0) ? 0 : cb;
     // at most `per` systems per group; the kernel balances the groups over the
     // systems each launch applies
-    const int per = ctx->loop_systems > 0 ? min(ctx->loop_systems, a.nrhs) : kLoopGroup;
+    const int per = wd_loop_group_cap(ctx, a.nrhs, a.chunk);
     const int grid = a.nsb * ((a.nrhs + per - 1) / per);
     // automatic prefetch: 1 ahead at 3 wavefronts per SIMD for the hop and M
     // launches; 3 ahead at 2 for the dot launches, which spill 20-23

@@ -22,7 +22,8 @@ struct SloppyArgs {
     const int *stop;         // whole launch becomes a no-op when *stop != 0
     int nrhs;
     float hc;                // hop coefficient: out = diag in + hc hop(in); -1/2 for M
-    int chunk;               // site blocks per L2 chunk (0: groups of one block consecutive)
+    int chunk;               // site blocks per L2 chunk (0: groups of one block consecutive);
+                             // WD_RHS_LOOP: systems per pass (0: the context's choice)
     float one;               // 1.0f, set by the launcher (an operand the compiler cannot fold)
     // filled by the launcher
     int groups, bpp, nsb;
@@ -39,6 +40,10 @@ int wd_launch_hop_sloppy(wd_ctx *ctx, SloppyArgs a, int spinor_prec, int gauge_p
                          int R, int dagger, int cb, int mode);
 int wd_launch_hop_sloppy_loop(wd_ctx *ctx, const SloppyArgs &a, const Geom &g, int dagger, int cb,
                               int mode);
+// Largest system group of a WD_RHS_LOOP launch over nrhs systems: `chunk`
+// (the site-chunk argument, which for this kernel counts systems per pass)
+// when > 0, else WD_OPT_LOOP_SYSTEMS when set, else 12.
+int wd_loop_group_cap(const wd_ctx *ctx, int nrhs, int chunk);
 // partial sums per workgroup of the HOP_M_DOT / HOP_M_DOT_NORM launches: the
 // WD_RHS_LOOP kernel writes one per wavefront, the others one per workgroup
 constexpr int kLoopWaves = kBlock / 64;

@@ -247,6 +247,12 @@ def test_tune_report(w):
         best8, chunk8, ms8 = ctx.tune_report(f8, Uh)
         assert (w.RHS_LOOP, 0) in ms8 and (w.RHS_LOOP, 16) not in ms8
         assert ms8[(best8, chunk8)] == min(ms8.values())
+        # 40 systems: the loop kernel's chunk candidates are systems per pass,
+        # 12 (chunk 0, 4 passes), 16 (3), 32 (2), 64 (1 pass): all timed
+        f40 = ctx.field(prec=w.F16, nrhs=40, data=(bs * 7)[:40])
+        best40, chunk40, ms40 = ctx.tune_report(f40, Uh)
+        assert {(w.RHS_LOOP, C) for C in w.TUNE_CHUNK} <= set(ms40)
+        assert ms40[(best40, chunk40)] == min(ms40.values())
 
 
 def test_sloppy_argument_errors(w):

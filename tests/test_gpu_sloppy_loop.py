@@ -168,6 +168,21 @@ def test_loop_groups_and_prefetch_bit_identical(w):
         with pytest.raises(w.WilsonError) as e:
             ctx.set_option(w.OPT_LOOP_SYSTEMS, -1)
         assert e.value.code == w.WD_EARG
+        # the site-chunk argument of the loop kernel counts systems per pass
+        ctx.set_option(w.OPT_LOOP_PREFETCH, 0)
+        ctx.set_option(w.OPT_LOOP_SYSTEMS, 0)
+        for chunk in (3, 16):
+            got = []
+            for dag in (False, True):
+                ctx.apply_M_sloppy(fout, fin, Us, masses, dag, w.RHS_LOOP, chunk)
+                got += [fout.download(r) for r in range(n)]
+            x = ctx.field(nrhs=n)
+            res = ctx.cg_mixed(x, fb, U, Us, masses, tol=1e-10, rhs_per_thread=w.RHS_LOOP,
+                               site_chunk=chunk, even_odd=True)
+            assert res["status"] == 0
+            assert all(s["site_chunk"] == chunk for s in res["systems"])
+            got += [x.download(r) for r in range(n)]
+            runs[("chunk", chunk)] = (got, [s["iterations"] for s in res["systems"]])
     base_got, base_its = runs[(0, 0)]
     for key, (got, its) in runs.items():
         assert its == base_its, key
