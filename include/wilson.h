@@ -181,10 +181,13 @@ int wd_cg_eo(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd_gauge *U, dou
  * or F16 12-real); FP64 is kept for reductions, the solution and the true
  * residual of the reliable updates. */
 /* rhs_per_thread value: every system of a site in one thread, one after the
- * other, with the site's links loaded once and kept in registers as packed
- * halves; SU(3) products by packed F16 dot products accumulated in FP32 (the
- * half spinor and the rebuilt third link row rounded to F16).  F16 spinors and
- * F16 12-real links only.  Results differ from R = 1, 2, 4 by those roundings. */
+ * other, with the site's links loaded once per pass and kept in registers.
+ * F16 spinors with 12-real links.  F16 links are kept as packed halves and
+ * the SU(3) products are packed F16 dot products accumulated in FP32 (the
+ * half spinor and the rebuilt third link row rounded to F16): results differ
+ * from R = 1, 2, 4 by those roundings.  F32 links are kept in FP32 with the
+ * arithmetic of R = 1, 2, 4: the operator's results are bit-identical to
+ * theirs. */
 #define WD_RHS_LOOP 64
 
 typedef struct wd_mixed_params {
@@ -266,12 +269,12 @@ int wd_time_sloppy(wd_ctx *ctx, wd_field *out, const wd_field *in, const wd_gaug
                    double *ms_per_launch);
 
 /* Candidates of the autotuner: rhs per thread {1, 2, 4, WD_RHS_LOOP} x site
- * chunk {0, 16, 32, 64}.  WD_RHS_LOOP is timed for F16 spinors with F16
- * 12-real links from 8 systems on, its chunk values counting systems per
+ * chunk {0, 16, 32, 64}.  WD_RHS_LOOP is timed for F16 spinors with F16 or
+ * F32 12-real links from 8 systems on, its chunk values counting systems per
  * pass (0 = WD_OPT_LOOP_SYSTEMS, 12 by default; values that give the same
- * number of passes are timed once); the other rhs-per-thread values give
- * bit-identical results, WD_RHS_LOOP differs by its F16 roundings (all its
- * pass sizes give the same bits).  The choice is cached per context. */
+ * number of passes are timed once).  The operator gives the same bits for
+ * every candidate except WD_RHS_LOOP with F16 links (its F16 roundings; all
+ * its pass sizes give the same bits).  The choice is cached per context. */
 #define WD_TUNE_NR 4
 #define WD_TUNE_NC 4
 

@@ -897,10 +897,12 @@ int poll_flag(wd_ctx *ctx, const int *stop, const char *who)
 // candidate and keeps the fastest per (nrhs, spinor format, link format,
 // one or both parities).
 // fixed_R > 0 restricts the search to that rhs-per-thread.  Every system of a
-// site in one thread (WD_RHS_LOOP) is a candidate for F16 spinors with F16
-// 12-real links from kLoopTuneMin systems on, where it shares each link load
-// among many systems; its products round the half spinor to F16, so below
-// that the candidates stay the bit-identical work orders.  For WD_RHS_LOOP the
+// site in one thread (WD_RHS_LOOP) is a candidate for F16 spinors with F16 or
+// F32 12-real links from kLoopTuneMin systems on, where it shares each link
+// load among many systems; with F16 links its products round the half spinor
+// to F16, so below that the candidates stay the bit-identical work orders
+// (with F32 links its operator is bit-identical to R = 1, 2, 4, but its dot
+// products sum per wavefront).  For WD_RHS_LOOP the
 // chunk candidates are systems per pass (0: the context's choice, 12 by
 // default), so the systems-per-thread axis runs 1, 2, 4 and 12 .. 64; group
 // sizes give bit-identical results.  The choice is cached per context, so the

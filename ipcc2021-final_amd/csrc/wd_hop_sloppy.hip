@@ -39,55 +39,6 @@ using namespace wdsl;
 
 namespace {
 
-// a + mulc<P>(b) from two packed complex, one v_fma_mix_f32 per real.  `one`
-// is 1.0f passed in from the host: with a literal 1 the compiler would fold
-// the fma into widen + add (three instructions instead of one).
-template <int P>
-__device__ __forceinline__ cpx<float> proj16(unsigned wa, unsigned wb, float one)
-{
-    if constexpr (P == 0)
-        return {fmaf(re16(wb), one, re16(wa)), fmaf(im16(wb), one, im16(wa))};
-    else if constexpr (P == 1)
-        return {fmaf(re16(wb), -one, re16(wa)), fmaf(im16(wb), -one, im16(wa))};
-    else if constexpr (P == 2) // +i b = (-b.i, b.r)
-        return {fmaf(im16(wb), -one, re16(wa)), fmaf(re16(wb), one, im16(wa))};
-    else // -i b = (b.i, -b.r)
-        return {fmaf(im16(wb), one, re16(wa)), fmaf(re16(wb), -one, im16(wa))};
-}
-
-// One link with 32-bit lane offsets (row 2 rebuilt for 12-real formats).  NT:
-// non-temporal loads, right when every link is read once per launch.
-template <int GF, bool NT>
-__device__ __forceinline__ void load_link32(const char *base, int stride, unsigned j,
-                                            cpx<float> (&U)[3][3])
-{
-    if constexpr (GF == G_F32_18) {
-#pragma unroll
-        for (int k = 0; k < 9; ++k) {
-            const f2v c = ldg<NT>(reinterpret_cast<const f2v *>(
-                base + ((unsigned)k * (unsigned)stride + j) * 8u));
-            U[k / 3][k % 3] = {c.x, c.y};
-        }
-    } else if constexpr (GF == G_F32_12) {
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            const f4v c = ldg<NT>(reinterpret_cast<const f4v *>(base + off16(q, stride, j)));
-            U[(2 * q) / 3][(2 * q) % 3] = {c.x, c.y};
-            U[(2 * q + 1) / 3][(2 * q + 1) % 3] = {c.z, c.w};
-        }
-        reconstruct_row2(U);
-    } else {
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            const u2v c = ldg<NT>(reinterpret_cast<const u2v *>(
-                base + ((unsigned)q * (unsigned)stride + j) * 8u));
-            U[(2 * q) / 3][(2 * q) % 3] = {re16(c.x), im16(c.x)};
-            U[(2 * q + 1) / 3][(2 * q + 1) % 3] = {re16(c.y), im16(c.y)};
-        }
-        reconstruct_row2(U);
-    }
-}
-
 template <int R>
 __device__ __forceinline__ void pin_accs(cpx<float> (&acc)[R][4][3])
 {
@@ -432,7 +383,8 @@ bool wd_sloppy_supported(int spinor_prec, int gauge_prec, int recon)
 
 bool wd_sloppy_loop_supported(int spinor_prec, int gauge_prec, int recon)
 {
-    return spinor_prec == WD_F16 && gauge_prec == WD_F16 && recon == WD_RECON_12;
+    return spinor_prec == WD_F16 && (gauge_prec == WD_F16 || gauge_prec == WD_F32) &&
+           recon == WD_RECON_12;
 }
 
 int wd_sloppy_groups(int nrhs, int R) { return R == WD_RHS_LOOP ? 1 : (nrhs + R - 1) / R; }
@@ -466,7 +418,7 @@ int wd_launch_hop_sloppy(wd_ctx *ctx, SloppyArgs a, int spinor_prec, int gauge_p
     const long grid = (long)a.nsb * a.groups;
     WD_ARG_CHECK(ctx, grid < (1L << 31), "sloppy hop: grid too large");
     if (R == WD_RHS_LOOP)
-        return wd_launch_hop_sloppy_loop(ctx, a, g, dagger, cb, mode);
+        return wd_launch_hop_sloppy_loop(ctx, a, g, dagger, cb, mode, gauge_prec);
     const int cb_first = (cb < 0) ? 0 : cb;
     const int gf = gauge_fmt(gauge_prec, recon);
     if (spinor_prec == WD_F16) {

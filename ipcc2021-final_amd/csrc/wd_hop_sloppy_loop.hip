@@ -197,6 +197,42 @@ __device__ __forceinline__ void dir_math(cpx<float> (&acc)[4][3], const unsigned
     pin_acc(acc);
 }
 
+// Link formats of the loop kernel: F16 12-real links kept as packed halves
+// (72 registers, products by packed dot products), or F32 12-real links kept
+// as FP32 complex with row 2 rebuilt (144 registers, the FP32 products of the
+// R-systems-per-thread kernel, whose results they reproduce bit for bit).
+enum LoopLinks { LL_F16 = 0, LL_F32 = 1 };
+
+template <int LF>
+struct LinkRegs;
+template <>
+struct LinkRegs<LL_F16> {
+    unsigned W[8][3][3];
+};
+template <>
+struct LinkRegs<LL_F32> {
+    cpx<float> W[8][3][3];
+};
+
+// acc += f * reconstruct(W h) or f * reconstruct(W^dag h) in FP32, h projected
+// from the F16 neighbour n: the per-direction arithmetic of the kernel with
+// R systems per thread (same operations in the same order).
+template <int MU, bool FWD, bool DAG>
+__device__ __forceinline__ void dir_math32(cpx<float> (&acc)[4][3], const cpx<float> (&W)[3][3],
+                                           const Nbr &n, float hs, float one)
+{
+    using D = Dir<MU, FWD, DAG>;
+    cpx<float> h[2][3], res[2][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        h[0][c] = proj16<D::P0>(n.w[c], n.w[3 * D::S0 + c], one);
+        h[1][c] = proj16<D::P1>(n.w[3 + c], n.w[3 * D::S1 + c], one);
+    }
+    su3_mul_half<!FWD>(W, h, res);
+    reconstruct_f<D>(acc, res, hs * n.s);
+    pin_acc(acc);
+}
+
 // Sum of (sr, si) over the wavefront, stored by lane 0: the per-system partials
 // of this kernel are per wavefront (kBlock / 64 per workgroup), so no
 // workgroup barrier is needed per system.
@@ -213,10 +249,10 @@ __device__ __forceinline__ void wave_sum_store(double sr, double si, double2 *ds
 
 // The 8 hop directions of one system, d = 2 mu + (backward), with the
 // neighbour loads K directions ahead in K + 1 rotating buffers.
-template <bool DAG, int MODE, int K>
+template <bool DAG, int MODE, int K, int LF>
 struct Walk {
     const SloppyArgs &a;
-    const unsigned (&W)[8][3][3];
+    const LinkRegs<LF> &L;
     const unsigned (&j)[8];
     const char *inb;
     const float *isc;
@@ -231,24 +267,33 @@ struct Walk {
             load_nbr(inb, isc, j[D + K], stride, nb[(D + K) % (K + 1)]);
         else if constexpr (D + K == 8 && MODE != HOP_ONLY)
             load_nbr(a.same + off, a.same_scale + soff, ui, stride, nb[8 % (K + 1)]);
-        dir_math<D / 2, D % 2 == 0, DAG>(acc, W[D], nb[D % (K + 1)], hop_factor(a.hc, sbits, D));
+        if constexpr (LF == LL_F16)
+            dir_math<D / 2, D % 2 == 0, DAG>(acc, L.W[D], nb[D % (K + 1)], hop_factor(a.hc, sbits, D));
+        else
+            dir_math32<D / 2, D % 2 == 0, DAG>(acc, L.W[D], nb[D % (K + 1)],
+                                               hop_factor(a.hc, sbits, D), a.one);
     }
 };
 
-// Prefetch depth and occupancy: K = 1 at 3 wavefronts per SIMD, K = 3 at 2.
-template <int K>
+// Prefetch depth and occupancy: F16 links K = 1 at 3 wavefronts per SIMD,
+// K = 3 at 2; F32 links (twice the link registers) K = 1 at 2.
+template <int K, int LF>
 struct LoopShape;
 template <>
-struct LoopShape<1> {
+struct LoopShape<1, LL_F16> {
     static constexpr int waves = 3;
 };
 template <>
-struct LoopShape<3> {
+struct LoopShape<3, LL_F16> {
+    static constexpr int waves = 2;
+};
+template <>
+struct LoopShape<1, LL_F32> {
     static constexpr int waves = 2;
 };
 
 // One site block (kBlock sites of one parity) for systems [r0, r1).
-template <bool DAG, int MODE, int K>
+template <bool DAG, int MODE, int K, int LF>
 __device__ __forceinline__ void loop_block(const SloppyArgs &a, const Geom &g, int sb, int cb_first,
                                            int nparity, int il, int r0, int r1)
 {
@@ -263,7 +308,7 @@ __device__ __forceinline__ void loop_block(const SloppyArgs &a, const Geom &g, i
     const size_t pbytes = (size_t)3 * 16 * stride;
 
     // the site's 8 links, neighbour offsets and hop factors, for all systems
-    unsigned W[8][3][3];
+    LinkRegs<LF> L;
     unsigned jn[8];
     unsigned sbits = 0; // bit d: hop d crosses a boundary with phase -1
     if (site_ok) {
@@ -294,18 +339,29 @@ __device__ __forceinline__ void loop_block(const SloppyArgs &a, const Geom &g, i
         for (int d = 0; d < 8; ++d)
             if (cross[d] && g.bc[d >> 1] < 0.0)
                 sbits |= 1u << d;
-        const size_t lblk = (size_t)3 * 8 * stride; // F16 12-real: 3 x 8 bytes per site
+        // 12-real links, 3 chunks per site: 8 bytes each (F16), 16 (F32)
+        const size_t lblk = (size_t)3 * (LF == LL_F16 ? 8 : 16) * stride;
 #pragma unroll
         for (int mu = 0; mu < 4; ++mu) {
             // forward: U_mu(x) at (p, i); backward: U_mu(x - mu) at (1 - p, j)
             const char *Uf = a.U + (size_t)(mu * 2 + p) * lblk;
             const char *Ub = a.U + (size_t)(mu * 2 + 1 - p) * lblk;
-            if (a.tlinks) {
-                load_link_h2<false>(Uf, stride, (unsigned)i, W[2 * mu]);
-                load_link_h2<false>(Ub, stride, jn[2 * mu + 1], W[2 * mu + 1]);
+            if constexpr (LF == LL_F16) {
+                if (a.tlinks) {
+                    load_link_h2<false>(Uf, stride, (unsigned)i, L.W[2 * mu]);
+                    load_link_h2<false>(Ub, stride, jn[2 * mu + 1], L.W[2 * mu + 1]);
+                } else {
+                    load_link_h2<true>(Uf, stride, (unsigned)i, L.W[2 * mu]);
+                    load_link_h2<true>(Ub, stride, jn[2 * mu + 1], L.W[2 * mu + 1]);
+                }
             } else {
-                load_link_h2<true>(Uf, stride, (unsigned)i, W[2 * mu]);
-                load_link_h2<true>(Ub, stride, jn[2 * mu + 1], W[2 * mu + 1]);
+                if (a.tlinks) {
+                    load_link32<G_F32_12, false>(Uf, stride, (unsigned)i, L.W[2 * mu]);
+                    load_link32<G_F32_12, false>(Ub, stride, jn[2 * mu + 1], L.W[2 * mu + 1]);
+                } else {
+                    load_link32<G_F32_12, true>(Uf, stride, (unsigned)i, L.W[2 * mu]);
+                    load_link32<G_F32_12, true>(Ub, stride, jn[2 * mu + 1], L.W[2 * mu + 1]);
+                }
             }
         }
     }
@@ -340,7 +396,7 @@ __device__ __forceinline__ void loop_block(const SloppyArgs &a, const Geom &g, i
             // the arithmetic of direction d; the site's own spinor (the
             // diagonal term) follows the last direction's loads
             Nbr nb[K + 1];
-            const Walk<DAG, MODE, K> walk{a, W, j, inb, isc, off, soff, ui, sbits, stride};
+            const Walk<DAG, MODE, K, LF> walk{a, L, j, inb, isc, off, soff, ui, sbits, stride};
 #pragma unroll
             for (int d = 0; d < K; ++d)
                 load_nbr(inb, isc, j[d], stride, nb[d]);
@@ -418,8 +474,8 @@ __device__ __forceinline__ int nth_set_bit(unsigned long long m, int j)
     return __ffsll((long long)__ballot(((m >> lane) & 1ull) && rank == j)) - 1;
 }
 
-template <bool DAG, int MODE, int K>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(LoopShape<K>::waves))) void
+template <bool DAG, int MODE, int K, int LF>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(LoopShape<K, LF>::waves))) void
 k_hop_sloppy_loop(SloppyArgs a, Geom g, int cb_first, int nparity, int remap, int per, int il)
 {
     if (a.stop && *a.stop)
@@ -462,30 +518,30 @@ k_hop_sloppy_loop(SloppyArgs a, Geom g, int cb_first, int nparity, int remap, in
             k += c;
         }
     }
-    loop_block<DAG, MODE, K>(a, g, sb, cb_first, nparity, il, r0, r1);
+    loop_block<DAG, MODE, K, LF>(a, g, sb, cb_first, nparity, il, r0, r1);
 }
 
-template <bool DAG, int K>
+template <bool DAG, int K, int LF>
 void launch_mode(wd_ctx *ctx, const SloppyArgs &a, const Geom &g, int cb_first, int nparity,
                  int mode, int grid, int per)
 {
     switch (mode) {
     case HOP_ONLY:
-        hipLaunchKernelGGL((k_hop_sloppy_loop<DAG, HOP_ONLY, K>), dim3(grid), dim3(kBlock), 0,
+        hipLaunchKernelGGL((k_hop_sloppy_loop<DAG, HOP_ONLY, K, LF>), dim3(grid), dim3(kBlock), 0,
                            ctx->stream, a, g, cb_first, nparity, ctx->xcd_remap, per, ctx->parity_il);
         break;
     case HOP_M:
-        hipLaunchKernelGGL((k_hop_sloppy_loop<DAG, HOP_M, K>), dim3(grid), dim3(kBlock), 0,
+        hipLaunchKernelGGL((k_hop_sloppy_loop<DAG, HOP_M, K, LF>), dim3(grid), dim3(kBlock), 0,
                            ctx->stream, a, g, cb_first, nparity, ctx->xcd_remap, per, ctx->parity_il);
         break;
     case HOP_M_DOT_NORM:
         if constexpr (!DAG)
-            hipLaunchKernelGGL((k_hop_sloppy_loop<false, HOP_M_DOT_NORM, K>), dim3(grid),
+            hipLaunchKernelGGL((k_hop_sloppy_loop<false, HOP_M_DOT_NORM, K, LF>), dim3(grid),
                                dim3(kBlock), 0, ctx->stream, a, g, cb_first, nparity,
                                ctx->xcd_remap, per, ctx->parity_il);
         break;
     default:
-        hipLaunchKernelGGL((k_hop_sloppy_loop<DAG, HOP_M_DOT, K>), dim3(grid), dim3(kBlock), 0,
+        hipLaunchKernelGGL((k_hop_sloppy_loop<DAG, HOP_M_DOT, K, LF>), dim3(grid), dim3(kBlock), 0,
                            ctx->stream, a, g, cb_first, nparity, ctx->xcd_remap, per, ctx->parity_il);
         break;
     }
@@ -509,7 +565,7 @@ int wd_loop_group_cap(const wd_ctx *ctx, int nrhs, int chunk)
 // Called by wd_launch_hop_sloppy (which has checked the arguments and filled
 // bpp, nsb, tlinks and one) for R = WD_RHS_LOOP.
 int wd_launch_hop_sloppy_loop(wd_ctx *ctx, const SloppyArgs &a, const Geom &g, int dagger,
-                              int cb, int mode)
+                              int cb, int mode, int gauge_prec)
 {
     const int nparity = (cb < 0) ? 2 : 1;
     const int cb_first = (cb < 0) ? 0 : cb;
@@ -517,6 +573,14 @@ int wd_launch_hop_sloppy_loop(wd_ctx *ctx, const SloppyArgs &a, const Geom &g, i
     // systems each launch applies
     const int per = wd_loop_group_cap(ctx, a.nrhs, a.chunk);
     const int grid = a.nsb * ((a.nrhs + per - 1) / per);
+    if (gauge_prec == WD_F32) {
+        // F32 links: one direction ahead at 2 wavefronts per SIMD for every mode
+        if (dagger)
+            launch_mode<true, 1, LL_F32>(ctx, a, g, cb_first, nparity, mode, grid, per);
+        else
+            launch_mode<false, 1, LL_F32>(ctx, a, g, cb_first, nparity, mode, grid, per);
+        return wd_check_launch(ctx, "k_hop_sloppy_loop");
+    }
     // automatic prefetch: 1 ahead at 3 wavefronts per SIMD for the hop and M
     // launches; 3 ahead at 2 for the dot launches, which spill 20-23
     // registers at 3 wavefronts per SIMD
@@ -524,14 +588,14 @@ int wd_launch_hop_sloppy_loop(wd_ctx *ctx, const SloppyArgs &a, const Geom &g, i
     const int K = ctx->loop_prefetch ? ctx->loop_prefetch : dot ? 3 : 1;
     if (K == 3) {
         if (dagger)
-            launch_mode<true, 3>(ctx, a, g, cb_first, nparity, mode, grid, per);
+            launch_mode<true, 3, LL_F16>(ctx, a, g, cb_first, nparity, mode, grid, per);
         else
-            launch_mode<false, 3>(ctx, a, g, cb_first, nparity, mode, grid, per);
+            launch_mode<false, 3, LL_F16>(ctx, a, g, cb_first, nparity, mode, grid, per);
     } else {
         if (dagger)
-            launch_mode<true, 1>(ctx, a, g, cb_first, nparity, mode, grid, per);
+            launch_mode<true, 1, LL_F16>(ctx, a, g, cb_first, nparity, mode, grid, per);
         else
-            launch_mode<false, 1>(ctx, a, g, cb_first, nparity, mode, grid, per);
+            launch_mode<false, 1, LL_F16>(ctx, a, g, cb_first, nparity, mode, grid, per);
     }
     return wd_check_launch(ctx, "k_hop_sloppy_loop");
 }

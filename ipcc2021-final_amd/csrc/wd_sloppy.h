@@ -39,7 +39,7 @@ int wd_sloppy_groups(int nrhs, int R);
 int wd_launch_hop_sloppy(wd_ctx *ctx, SloppyArgs a, int spinor_prec, int gauge_prec, int recon,
                          int R, int dagger, int cb, int mode);
 int wd_launch_hop_sloppy_loop(wd_ctx *ctx, const SloppyArgs &a, const Geom &g, int dagger, int cb,
-                              int mode);
+                              int mode, int gauge_prec);
 // Largest system group of a WD_RHS_LOOP launch over nrhs systems: `chunk`
 // (the site-chunk argument, which for this kernel counts systems per pass)
 // when > 0, else WD_OPT_LOOP_SYSTEMS when set, else 12.

@@ -2,8 +2,10 @@
 // operators (wd_hop_sloppy.hip: R systems per thread; wd_hop_sloppy_loop.hip:
 // every system of a site in one thread): 32-bit lane offsets into the
 // [chunks][stride] spinor arrays, F16 site loads and stores with the per-site
-// scale, the reconstruction of a direction's half spinor into the 4-spinor
-// accumulator, and the workgroup reduction of the per-system partials.
+// scale, FP32 link loads (row 2 rebuilt for 12-real formats), the projection
+// of an F16 neighbour into a half spinor, the reconstruction of a direction's
+// half spinor into the 4-spinor accumulator, and the workgroup reduction of
+// the per-system partials.
 #pragma once
 
 #include "wd_fields.h"
@@ -41,6 +43,55 @@ __device__ __forceinline__ void load32(const char *base, int stride, unsigned j,
         const f4v c = *reinterpret_cast<const f4v *>(base + off16(q, stride, j));
         v[(2 * q) / 3][(2 * q) % 3] = {c.x, c.y};
         v[(2 * q + 1) / 3][(2 * q + 1) % 3] = {c.z, c.w};
+    }
+}
+
+// a + mulc<P>(b) from two packed complex, one v_fma_mix_f32 per real.  `one`
+// is 1.0f passed in from the host: with a literal 1 the compiler would fold
+// the fma into widen + add (three instructions instead of one).
+template <int P>
+__device__ __forceinline__ cpx<float> proj16(unsigned wa, unsigned wb, float one)
+{
+    if constexpr (P == 0)
+        return {fmaf(re16(wb), one, re16(wa)), fmaf(im16(wb), one, im16(wa))};
+    else if constexpr (P == 1)
+        return {fmaf(re16(wb), -one, re16(wa)), fmaf(im16(wb), -one, im16(wa))};
+    else if constexpr (P == 2) // +i b = (-b.i, b.r)
+        return {fmaf(im16(wb), -one, re16(wa)), fmaf(re16(wb), one, im16(wa))};
+    else // -i b = (b.i, -b.r)
+        return {fmaf(im16(wb), one, re16(wa)), fmaf(re16(wb), -one, im16(wa))};
+}
+
+// One link with 32-bit lane offsets (row 2 rebuilt for 12-real formats).  NT:
+// non-temporal loads, right when every link is read once per launch.
+template <int GF, bool NT>
+__device__ __forceinline__ void load_link32(const char *base, int stride, unsigned j,
+                                            cpx<float> (&U)[3][3])
+{
+    if constexpr (GF == G_F32_18) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            const f2v c = ldg<NT>(reinterpret_cast<const f2v *>(
+                base + ((unsigned)k * (unsigned)stride + j) * 8u));
+            U[k / 3][k % 3] = {c.x, c.y};
+        }
+    } else if constexpr (GF == G_F32_12) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const f4v c = ldg<NT>(reinterpret_cast<const f4v *>(base + off16(q, stride, j)));
+            U[(2 * q) / 3][(2 * q) % 3] = {c.x, c.y};
+            U[(2 * q + 1) / 3][(2 * q + 1) % 3] = {c.z, c.w};
+        }
+        reconstruct_row2(U);
+    } else {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const u2v c = ldg<NT>(reinterpret_cast<const u2v *>(
+                base + ((unsigned)q * (unsigned)stride + j) * 8u));
+            U[(2 * q) / 3][(2 * q) % 3] = {re16(c.x), im16(c.x)};
+            U[(2 * q + 1) / 3][(2 * q + 1) % 3] = {re16(c.y), im16(c.y)};
+        }
+        reconstruct_row2(U);
     }
 }
 

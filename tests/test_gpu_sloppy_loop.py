@@ -1,6 +1,7 @@
 """The sloppy operator with every system of a site in one thread
-(rhs_per_thread = WD_RHS_LOOP: links loaded once per site and kept as packed
-halves, SU(3) products by packed F16 dot products with FP32 accumulation).
+(rhs_per_thread = WD_RHS_LOOP: links loaded once per site; F16 links kept as
+packed halves, SU(3) products by packed F16 dot products with FP32
+accumulation; F32 links kept as FP32, the arithmetic of the R kernels).
 
 Tolerances: against the FP64 oracle the same 2e-3 relative L2 as the other
 F16-spinor formats; against the R = 1 kernel (FP32 products of the same F16
@@ -99,10 +100,10 @@ def test_loop_graph_and_direct_launch_agree(w):
 def test_loop_argument_errors(w):
     L = [4, 4, 4, 4]
     with w.Context(L) as ctx:
-        U32 = ctx.gauge(w.gauge_unit(L), prec=w.F32, recon=12)
+        U32 = ctx.gauge(w.gauge_unit(L), prec=w.F32, recon=18)
         f16 = ctx.field(prec=w.F16, nrhs=2)
         o16 = ctx.field(prec=w.F16, nrhs=2)
-        with pytest.raises(w.WilsonError) as e:   # F32 links: not a loop-kernel format
+        with pytest.raises(w.WilsonError) as e:   # 18-real links: not a loop-kernel format
             ctx.apply_M_sloppy(o16, f16, U32, [0.1, 0.1], rhs_per_thread=w.RHS_LOOP)
         assert e.value.code == w.WD_EARG
         U = ctx.gauge(w.gauge_unit(L))
@@ -187,3 +188,42 @@ def test_loop_groups_and_prefetch_bit_identical(w):
     for key, (got, its) in runs.items():
         assert its == base_its, key
         assert all(np.array_equal(a, b) for a, b in zip(got, base_got)), key
+
+
+@pytest.mark.parametrize("L", [[4, 4, 6, 8], [6, 4, 6, 10]], ids=lambda L: "x".join(map(str, L)))
+def test_loop_f32_links_bit_identical_to_r_kernels(w, oracle, L):
+    """With F32 12-real links the loop kernel keeps the links in FP32 and does
+    the FP32 arithmetic of the R-systems-per-thread kernel, so M and M^dagger
+    are bit-identical to R = 1, 2, 4 for every group size; a Schur CG on it
+    reaches the oracle's FP64 residual (its dot products sum per wavefront,
+    so its iterates may differ from R = 1 by roundings)."""
+    n = 9
+    A, bs = _setup(w, L, n)
+    masses = [kappa_to_mass(0.11 + 0.003 * r) for r in range(n)]
+    with w.Context(L, bc=[1, 1, 1, -1]) as ctx:
+        U = ctx.gauge(A)
+        Us = ctx.gauge(A, prec=w.F32, recon=12)
+        fin = ctx.field(prec=w.F16, nrhs=n, data=bs)
+        fout = ctx.field(prec=w.F16, nrhs=n)
+        for dag in (False, True):
+            ref = None
+            for R, chunk in ((1, 0), (2, 0), (4, 16), (w.RHS_LOOP, 0), (w.RHS_LOOP, 4),
+                             (w.RHS_LOOP, 64)):
+                ctx.apply_M_sloppy(fout, fin, Us, masses, dag, R, chunk)
+                got = [fout.download(r) for r in range(n)]
+                if ref is None:
+                    ref = got
+                assert all(np.array_equal(a, b) for a, b in zip(got, ref)), (dag, R, chunk)
+        fb = ctx.field(nrhs=n, data=bs)
+        x = ctx.field(nrhs=n)
+        res = ctx.cg_mixed(x, fb, U, Us, masses, tol=1e-10, rhs_per_thread=w.RHS_LOOP,
+                           even_odd=True, check_every=4)
+        xs = [x.download(r) for r in range(n)]
+    assert res["status"] == 0
+    assert all(s["rhs_per_thread"] == w.RHS_LOOP for s in res["systems"])
+    Af = [a.copy() for a in A]
+    w.gauge_flip_boundary(L, 3, Af)
+    for r in (0, n - 1):
+        Mdb = oracle.dslash(bs[r], Af, L, masses[r], 1)
+        rr = Mdb - oracle.dslash(oracle.dslash(xs[r], Af, L, masses[r], 0), Af, L, masses[r], 1)
+        assert np.linalg.norm(rr) / np.linalg.norm(Mdb) < 1.05e-10
