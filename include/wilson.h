@@ -221,6 +221,9 @@ typedef struct wd_mixed_report {
     int rhs_per_thread;   /* the rhs-per-thread the operator ran with */
     int site_chunk;       /* the L2 chunk the operator ran with (WD_RHS_LOOP: systems
                              per pass, 0 = the context's choice) */
+    int block_tile;       /* the operator's work order: 0 = z-t tiled columns
+                             (WD_OPT_BLOCK_TILE), -1 = natural site order (chosen by
+                             the autotuner) */
 } wd_mixed_report;
 
 /* max_iter 100000, tol 1e-10, delta 2e-2, F16 spinors, autotuned rhs per
@@ -282,7 +285,10 @@ int wd_time_sloppy(wd_ctx *ctx, wd_field *out, const wd_field *in, const wd_gaug
  * (ms[ir * WD_TUNE_NC + ic], HIP events on the context stream; candidates that
  * cannot differ from chunk 0 because f has one thread group are not timed and
  * read -1) and records the fastest (*best_R, *best_chunk) for later solves of
- * that shape. */
+ * that shape.  Each candidate runs in two work orders, the z-t tiled columns
+ * and the natural site order (when they differ); ms holds the faster, and the
+ * solvers use the faster (wd_mixed_report.block_tile).  Work orders give
+ * bit-identical results. */
 int wd_tune_report(wd_ctx *ctx, const wd_field *f, const wd_gauge *Us, int *best_R,
                    int *best_chunk, double ms[WD_TUNE_NR * WD_TUNE_NC]);
 

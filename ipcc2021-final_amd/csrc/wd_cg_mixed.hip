@@ -912,22 +912,28 @@ static const int kTuneC[WD_TUNE_NC] = {0, 16, 32, 64};
 constexpr int kLoopTuneMin = 8;
 
 int wd_tune_sloppy(wd_ctx *ctx, const SloppyArgs &a0, int sprec, int gprec, int recon, int cb,
-                   int fixed_R, int *best_R, int *best_C, double *ms_out)
+                   int fixed_R, int *best_R, int *best_C, int *best_T, double *ms_out)
 {
     // cb = -1: time the both-parity M (full-system solver); cb = 0: the
     // one-parity launch of the Schur solver, whose best work order differs
     const auto key = std::make_tuple(a0.nrhs, sprec, gprec * 100 + recon + (cb < 0 ? 0 : 1000));
     auto it = ctx->tuned.find(key);
-    if (it != ctx->tuned.end() && !ms_out && (fixed_R <= 0 || it->second.first == fixed_R)) {
-        *best_R = it->second.first;
-        *best_C = it->second.second;
+    if (it != ctx->tuned.end() && !ms_out && (fixed_R <= 0 || std::get<0>(it->second) == fixed_R)) {
+        *best_R = std::get<0>(it->second);
+        *best_C = std::get<1>(it->second);
+        if (best_T)
+            *best_T = std::get<2>(it->second);
         return WD_OK;
     }
     if (ms_out)
         for (int k = 0; k < WD_TUNE_NR * WD_TUNE_NC; ++k)
             ms_out[k] = -1.0;
     double best = 1e30;
-    int bestR = fixed_R > 0 ? fixed_R : 1, bestC = 0;
+    int bestR = fixed_R > 0 ? fixed_R : 1, bestC = 0, bestT = 0;
+    // work orders per candidate: the z-t tiled columns, and the natural site
+    // order when the tiled one differs from it (one system at 32^3x64: the
+    // one-parity launch 75 -> 72 us with F16 links, 118 -> 107 us with F32)
+    const int ntiles = ctx->geom.tW > 0 ? 2 : 1;
     hipEvent_t e0, e1;
     WD_HIP_CHECK(ctx, hipEventCreate(&e0));
     WD_HIP_CHECK(ctx, hipEventCreate(&e1));
@@ -954,27 +960,31 @@ int wd_tune_sloppy(wd_ctx *ctx, const SloppyArgs &a0, int sprec, int gprec, int 
                 if (ng <= 31)
                     loop_groups_seen |= 1 << (ng - 1);
             }
-            SloppyArgs a = a0;
-            a.chunk = kTuneC[ic];
-            for (int w = 0; w < 2 && !rc; ++w)
-                rc = wd_launch_hop_sloppy(ctx, a, sprec, gprec, recon, R, 0, cb, HOP_M);
-            if (rc)
-                break;
-            (void)hipEventRecord(e0, ctx->stream);
-            const int reps = 5;
-            for (int w = 0; w < reps && !rc; ++w)
-                rc = wd_launch_hop_sloppy(ctx, a, sprec, gprec, recon, R, 0, cb, HOP_M);
-            (void)hipEventRecord(e1, ctx->stream);
-            (void)hipEventSynchronize(e1);
-            float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, e0, e1);
-            const double t = ms / reps;
-            if (ms_out)
-                ms_out[ir * WD_TUNE_NC + ic] = t;
-            if (t < best) {
-                best = t;
-                bestR = R;
-                bestC = kTuneC[ic];
+            for (int itile = 0; itile < ntiles && !rc; ++itile) {
+                SloppyArgs a = a0;
+                a.chunk = kTuneC[ic];
+                a.tile = -itile; // 0: tiled columns, -1: natural order
+                for (int w = 0; w < 2 && !rc; ++w)
+                    rc = wd_launch_hop_sloppy(ctx, a, sprec, gprec, recon, R, 0, cb, HOP_M);
+                if (rc)
+                    break;
+                (void)hipEventRecord(e0, ctx->stream);
+                const int reps = 5;
+                for (int w = 0; w < reps && !rc; ++w)
+                    rc = wd_launch_hop_sloppy(ctx, a, sprec, gprec, recon, R, 0, cb, HOP_M);
+                (void)hipEventRecord(e1, ctx->stream);
+                (void)hipEventSynchronize(e1);
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                const double t = ms / reps;
+                if (ms_out && (itile == 0 || t < ms_out[ir * WD_TUNE_NC + ic]))
+                    ms_out[ir * WD_TUNE_NC + ic] = t;
+                if (t < best) {
+                    best = t;
+                    bestR = R;
+                    bestC = kTuneC[ic];
+                    bestT = a.tile;
+                }
             }
         }
     }
@@ -983,9 +993,11 @@ int wd_tune_sloppy(wd_ctx *ctx, const SloppyArgs &a0, int sprec, int gprec, int 
     if (rc)
         return rc;
     if (fixed_R <= 0)
-        ctx->tuned[key] = std::make_pair(bestR, bestC);
+        ctx->tuned[key] = std::make_tuple(bestR, bestC, bestT);
     *best_R = bestR;
     *best_C = bestC;
+    if (best_T)
+        *best_T = bestT;
     return WD_OK;
 }
 
@@ -1175,13 +1187,15 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
         T.out_scale = qq.f->scale;
         T.in = T.same = (const char *)p.f->data;
         T.in_scale = T.same_scale = p.f->scale;
-        int tR = 0, tC = 0;
+        int tR = 0, tC = 0, tT = 0;
         if ((rc = wd_tune_sloppy(ctx, T, sp, Us->prec, Us->recon, eo ? 0 : -1, R, &tR, &tC,
-                                 nullptr)))
+                                 &tT, nullptr)))
             return rc;
         R = tR;
-        if (C < 0)
+        if (C < 0) {
             C = tC;
+            A.tile = tT; // the work order timed with that chunk
+        }
     }
     A.chunk = C;
     const int nsb = hop_nsb * wd_sloppy_partials_per_block(R); // partials of the operator
@@ -1420,6 +1434,7 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
             reports[r].rhs_norm = bn;
             reports[r].rhs_per_thread = R;
             reports[r].site_chunk = C;
+            reports[r].block_tile = A.tile;
         }
         all_conv = all_conv && conv;
         breakdown = breakdown || hs[r].status == 2;
@@ -1581,12 +1596,15 @@ extern "C" int wd_bicgstab_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, co
         T.out_scale = v.f->scale;
         T.in = T.same = (const char *)pv.f->data;
         T.in_scale = T.same_scale = pv.f->scale;
-        int tR = 0, tC = 0;
-        if ((rc = wd_tune_sloppy(ctx, T, sp, Us->prec, Us->recon, 0, R, &tR, &tC, nullptr)))
+        int tR = 0, tC = 0, tT = 0;
+        if ((rc = wd_tune_sloppy(ctx, T, sp, Us->prec, Us->recon, 0, R, &tR, &tC, &tT,
+                                 nullptr)))
             return rc;
         R = tR;
-        if (C < 0)
+        if (C < 0) {
             C = tC;
+            As.tile = tT; // the work order timed with that chunk
+        }
     }
     As.chunk = C;
     const int op_parts = hop_nsb * wd_sloppy_partials_per_block(R); // partials of the operator
@@ -1799,6 +1817,7 @@ extern "C" int wd_bicgstab_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, co
             reports[k].rhs_norm = bn;
             reports[k].rhs_per_thread = R;
             reports[k].site_chunk = C;
+            reports[k].block_tile = As.tile;
         }
         all_conv = all_conv && conv;
         breakdown = breakdown || hs[k].status == 2;
@@ -1835,7 +1854,8 @@ extern "C" int wd_tune_report(wd_ctx *ctx, const wd_field *f, const wd_gauge *Us
     a.diag = (const float *)diag.p;
     a.nrhs = f->nrhs;
     a.hc = -0.5f;
-    rc = wd_tune_sloppy(ctx, a, f->prec, Us->prec, Us->recon, -1, 0, best_R, best_chunk, ms);
+    rc = wd_tune_sloppy(ctx, a, f->prec, Us->prec, Us->recon, -1, 0, best_R, best_chunk, nullptr,
+                        ms);
     wd_field_destroy(out);
     return rc;
 }

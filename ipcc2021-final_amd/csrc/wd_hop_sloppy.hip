@@ -393,9 +393,12 @@ int wd_launch_hop_sloppy(wd_ctx *ctx, SloppyArgs a, int spinor_prec, int gauge_p
                          int R, int dagger, int cb, int mode)
 {
     // columns of the z-t tiled order twice as wide as the FP64 kernels' (the
-    // FP32/F16 working set per site is smaller; measured faster on MI355X)
+    // FP32/F16 working set per site is smaller; measured faster on MI355X),
+    // or the natural site order when the autotuner found it faster (a.tile < 0)
     Geom g = ctx->geom;
-    if (g.tW > 0)
+    if (a.tile < 0)
+        g.tW = 0;
+    else if (g.tW > 0)
         g.tW = 2 * g.tW < g.tD ? 2 * g.tW : 0;
     WD_ARG_CHECK(ctx, R == 1 || R == 2 || R == 4 || R == WD_RHS_LOOP,
                  "sloppy hop: rhs per thread must be 1, 2, 4 or WD_RHS_LOOP");

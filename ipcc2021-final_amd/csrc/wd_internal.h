@@ -58,10 +58,10 @@ struct wd_ctx {
     int ts_tb;            // t-run length of the t-streaming both-parity FP64 kernel, 0 = off
     int f64_pairs;        // 1: batched FP64 hops of several systems run two systems per thread
     int loop_prefetch;    // WD_RHS_LOOP kernel: neighbour loads 1 (3 wavefronts/SIMD) or 3 (2) ahead, 0 = by mode
-    int loop_systems;     // WD_RHS_LOOP kernel: systems per workgroup (0 = all)
-    // autotuned (rhs per thread, chunk) of the sloppy operator, keyed by
-    // (nrhs, spinor precision, 100 * gauge precision + recon)
-    std::map<std::tuple<int, int, int>, std::pair<int, int>> tuned;
+    int loop_systems;     // WD_RHS_LOOP kernel: systems per pass (0 = automatic)
+    // autotuned (rhs per thread, chunk, tile) of the sloppy operator, keyed by
+    // (nrhs, spinor precision, 100 * gauge precision + recon + 1000 for one parity)
+    std::map<std::tuple<int, int, int>, std::tuple<int, int, int>> tuned;
     // solver work blocks handed back by finished solves, (bytes, pointer),
     // reused by later solves that ask for the same size
     std::vector<std::pair<size_t, void *>> work;

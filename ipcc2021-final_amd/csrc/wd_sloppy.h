@@ -25,6 +25,7 @@ struct SloppyArgs {
     int chunk;               // site blocks per L2 chunk (0: groups of one block consecutive);
                              // WD_RHS_LOOP: systems per pass (0: the context's choice)
     float one;               // 1.0f, set by the launcher (an operand the compiler cannot fold)
+    int tile;                // work order: 0 = z-t tiled columns (the context's), -1 = natural order
     // filled by the launcher
     int groups, bpp, nsb;
     int tlinks;              // temporal link loads (links re-read within the launch)

@@ -116,6 +116,7 @@ class MixedReport(ctypes.Structure):
         ("rhs_norm", ctypes.c_double),
         ("rhs_per_thread", ctypes.c_int),
         ("site_chunk", ctypes.c_int),
+        ("block_tile", ctypes.c_int),
     ]
 
 
@@ -431,7 +432,7 @@ class Context:
             "systems": [{"iterations": r.iterations, "reliable_updates": r.reliable_updates,
                          "converged": bool(r.converged), "true_rel_rsd": r.true_rel_rsd,
                          "rhs_norm": r.rhs_norm, "rhs_per_thread": r.rhs_per_thread,
-                         "site_chunk": r.site_chunk}
+                         "site_chunk": r.site_chunk, "block_tile": r.block_tile}
                         for r in reps],
         }
 

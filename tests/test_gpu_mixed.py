@@ -274,3 +274,36 @@ def test_sloppy_argument_errors(w):
         with pytest.raises(w.WilsonError):
             ctx.cg_mixed(out, f64, ctx.gauge(w.gauge_unit(L)), Us, [0.1, 0.1],
                          reliable_every=-1)
+
+
+def test_sloppy_work_orders_bit_identical(w):
+    """The autotuner's work-order axis: the sloppy operator gives the same bits
+    in the z-t tiled order and in the natural site order, for every kernel
+    shape; the solvers report the order they ran with."""
+    L = [8, 8, 24, 8]   # a time slice of one parity = 3 site blocks: tiled columns of 2
+    n = 9
+    _, bs = _setup(w, L, n)
+    A = w.gauge_near_identity(L, 5, 0.3)
+    masses = [0.1 + 0.01 * r for r in range(n)]
+    with w.Context(L, bc=[1, 1, 1, -1]) as ctx:
+        tile0 = ctx.get_option(w.OPT_BLOCK_TILE)
+        assert tile0 > 0
+        fin = ctx.field(prec=w.F16, nrhs=n, data=bs)
+        fout = ctx.field(prec=w.F16, nrhs=n)
+        for gp in (w.F32, w.F16):
+            Us = ctx.gauge(A, prec=gp, recon=12)
+            for R in (1, 2, w.RHS_LOOP):
+                outs = []
+                for tile in (tile0, 0):
+                    ctx.set_option(w.OPT_BLOCK_TILE, tile)
+                    ctx.apply_M_sloppy(fout, fin, Us, masses, False, R, 0)
+                    outs.append([fout.download(r) for r in range(n)])
+                ctx.set_option(w.OPT_BLOCK_TILE, tile0)
+                assert all(np.array_equal(a, b) for a, b in zip(*outs)), (gp, R)
+        U = ctx.gauge(A)
+        Us = ctx.gauge(A, prec=w.F16, recon=12)
+        fb = ctx.field(nrhs=n, data=bs)
+        x = ctx.field(nrhs=n)
+        res = ctx.cg_mixed(x, fb, U, Us, masses, tol=1e-10, even_odd=True)
+        assert res["status"] == 0
+        assert all(s["block_tile"] in (0, -1) for s in res["systems"])
