@@ -664,8 +664,8 @@ def large_case(w, full_system=False):
     return out
 
 
-PROFILE_CASES = ("mixed32", "eo32", "bicgstab32", "batched48", "batched48_bicgstab", "large12",
-                 "sloppy48")
+PROFILE_CASES = ("mixed32", "eo32", "bicgstab32", "batched32", "batched48", "batched48_bicgstab",
+                 "large12", "sloppy48")
 
 
 def profile_case(w, name, R=None):
@@ -675,6 +675,8 @@ def profile_case(w, name, R=None):
                            (F16 spinors, F32 12-real links), delta 1e-2
       eo32 / bicgstab32    the same system, mixed CG on the Schur system / mixed
                            BiCGStab (F16 spinors and F16 12-real links)
+      batched32            32^3x64, 12 Gaussian systems, mixed CG on M^dag M
+                           (F16 spinors, F32 12-real links), delta 1e-2
       batched48            24^3x48, 48 point-source systems at 4 kappas, mixed
                            CG on the Schur system (F16 spinors and links)
       batched48_bicgstab   the same systems, mixed BiCGStab
@@ -687,6 +689,9 @@ def profile_case(w, name, R=None):
     if name in ("mixed32", "eo32", "bicgstab32"):
         L, masses = [32, 32, 32, 64], [kappa_mass(0.135)]
         srcs = [w.spinor_gaussian(L, 20211026)]
+    elif name == "batched32":
+        L, masses = [32, 32, 32, 64], [kappa_mass(0.135)] * 12
+        srcs = [w.spinor_gaussian(L, 20211026, r) for r in range(12)]
     elif name in ("batched48", "batched48_bicgstab", "sloppy48"):
         L = [24, 24, 24, 48]
         rng = np.random.RandomState(20211027)
@@ -701,7 +706,7 @@ def profile_case(w, name, R=None):
     A = w.gauge_near_identity(L, 20211025, 0.3)
     with w.Context(L, bc=[1, 1, 1, -1]) as ctx:
         U = ctx.gauge(A)
-        Us = ctx.gauge(A, prec=w.F32 if name == "mixed32" else w.F16, recon=12)
+        Us = ctx.gauge(A, prec=w.F32 if name in ("mixed32", "batched32") else w.F16, recon=12)
         if srcs is None:
             fb = ctx.field(nrhs=n)
             for r in range(n):
@@ -729,7 +734,8 @@ def profile_case(w, name, R=None):
                                          rhs_per_thread=R, site_chunk=0)
             else:
                 res = ctx.cg_mixed(x, fb, U, Us, masses, tol=1e-10, delta=DELTA,
-                                   rhs_per_thread=R, site_chunk=0, even_odd=name != "mixed32")
+                                   rhs_per_thread=R, site_chunk=0,
+                                   even_odd=name not in ("mixed32", "batched32"))
             require_converged(res, name)
             out = {"case": name, "systems": n, "rhs_per_thread": R,
                    "seconds": round(res["seconds"], 4),
