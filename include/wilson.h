@@ -380,7 +380,8 @@ int wd_link_product(wd_ctx *ctx, int impl, int nsites, int nsys, const uint32_t 
  * wavefronts per SIMD; 3 = the third next direction's, at 2 wavefronts per
  * SIMD; 0 (default) = 1 for the hop and M launches, 3 for the launches fused
  * with dot products (which would spill registers at 3 wavefronts per SIMD).
- * Results are bit-identical. */
+ * With F32 links the kernel always loads one direction ahead (2 wavefronts
+ * per SIMD).  Results are bit-identical. */
 #define WD_OPT_LOOP_PREFETCH 6
 /* Systems per workgroup of the WD_RHS_LOOP sloppy kernel: the systems run in
  * groups, each group a separate run of workgroups over all sites (group 0 for
