@@ -17,9 +17,9 @@
 // host only polls the flags every `check_every` iterations and the sequence
 // can be replayed from a captured hipGraph.  An iteration:
 //   xpay    p = r_s + beta p
-//   M       qq = M p                      (sloppy, all iterating systems in one launch)
-//   M^dag   q = M^dag qq, partial (p, q)  (sloppy)
-//   alpha   alpha = rho / (p, q)
+//   M       qq = M p, partial |qq|^2       (sloppy, all iterating systems in one launch)
+//   M^dag   q = M^dag qq                   (sloppy)
+//   alpha   alpha = rho / |qq|^2            ((p, M^dag M p) = |M p|^2, as in CGLS)
 //   update  x_s += alpha p, r_s -= alpha q, partial |r_s|^2
 //   decide  reliable update due?  then the system waits (active = 2), else
 //           beta = |r_s|^2 / rho, rho = |r_s|^2
@@ -1239,8 +1239,10 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
                                (const int *)active);
         if ((e = wd_check_launch(ctx, "k_mx_xpay")))
             return e;
-        if ((e = sloppy_op(0, qq.f, p.f, HOP_M, nullptr)) ||
-            (e = sloppy_op(1, q.f, qq.f, HOP_M_DOT, p.f)))
+        // (p, q) = (p, A^dag A p) = |A p|^2, summed over the stored A p by the
+        // launch that writes it, so the A^dag launch reads no third field
+        if ((e = sloppy_op(0, qq.f, p.f, HOP_M_NORM, nullptr)) ||
+            (e = sloppy_op(1, q.f, qq.f, HOP_M, nullptr)))
             return e;
         hipLaunchKernelGGL(k_mx_alpha, dim3(n), dim3(1024), 0, ctx->stream,
                            (const double2 *)part, nsb, st, active);

@@ -29,6 +29,8 @@
 //             exact in FP32; FP64 across sites)
 //   HOP_M_DOT_NORM  HOP_M_DOT plus the partials of |out|^2 (BiCGStab's |t|^2
 //             from the launch that writes t; no dagger form)
+//   HOP_M_NORM      HOP_M plus the partials (|out|^2, 0) (the CG's |A p|^2 from
+//             the launch that writes A p; no dagger form)
 #include "wd_fields.h"
 #include "wd_internal.h"
 #include "wd_sloppy.h"
@@ -247,7 +249,7 @@ __global__ __launch_bounds__(kBlock) void k_hop_sloppy(SloppyArgs a, Geom g, int
             if constexpr (SF == WD_F16) {
                 unsigned q[12];
                 const float sq = store16(a.out + off, a.out_scale + soff, stride, ui, acc[r], q);
-                if constexpr (MODE == HOP_M_DOT_NORM) {
+                if constexpr (MODE == HOP_M_DOT_NORM || MODE == HOP_M_NORM) {
                     // |q|^2 of the stored values: f16 squares are exact in FP32
                     float nq = 0.f;
 #pragma unroll
@@ -277,7 +279,7 @@ __global__ __launch_bounds__(kBlock) void k_hop_sloppy(SloppyArgs a, Geom g, int
                 }
             } else {
                 store32(a.out + off, stride, ui, acc[r]);
-                if constexpr (MODE == HOP_M_DOT_NORM) {
+                if constexpr (MODE == HOP_M_DOT_NORM || MODE == HOP_M_NORM) {
 #pragma unroll
                     for (int s = 0; s < 4; ++s)
 #pragma unroll
@@ -308,6 +310,8 @@ __global__ __launch_bounds__(kBlock) void k_hop_sloppy(SloppyArgs a, Geom g, int
             block_sum_store(pr, pi, a.partials + (size_t)rhs[r] * a.nsb + nat);
         if constexpr (MODE == HOP_M_DOT_NORM)
             block_sum_store(nn, 0.0, a.partials2 + (size_t)rhs[r] * a.nsb + nat);
+        if constexpr (MODE == HOP_M_NORM)
+            block_sum_store(nn, 0.0, a.partials + (size_t)rhs[r] * a.nsb + nat);
     }
 }
 
@@ -328,6 +332,13 @@ void launch_r(wd_ctx *ctx, const SloppyArgs &a, const Geom &g, int cb_first, int
         // (s, t) and |t|^2 of BiCGStab's t = A s: no dagger form is needed
         if constexpr (!DAG)
             hipLaunchKernelGGL((k_hop_sloppy<SF, GF, R, false, HOP_M_DOT_NORM>), dim3(grid),
+                               dim3(kBlock), 0, ctx->stream, a, g, cb_first, nparity,
+                               ctx->xcd_remap);
+        break;
+    case HOP_M_NORM:
+        // |A p|^2 of the CG's A p: no dagger form is needed
+        if constexpr (!DAG)
+            hipLaunchKernelGGL((k_hop_sloppy<SF, GF, R, false, HOP_M_NORM>), dim3(grid),
                                dim3(kBlock), 0, ctx->stream, a, g, cb_first, nparity,
                                ctx->xcd_remap);
         break;
@@ -410,6 +421,8 @@ int wd_launch_hop_sloppy(wd_ctx *ctx, SloppyArgs a, int spinor_prec, int gauge_p
     WD_ARG_CHECK(ctx, a.hc != 0.f, "sloppy hop: hop coefficient not set");
     WD_ARG_CHECK(ctx, mode != HOP_M_DOT_NORM || (!dagger && a.partials2),
                  "sloppy hop: the norm mode has no dagger form and needs partials2");
+    WD_ARG_CHECK(ctx, mode != HOP_M_NORM || (!dagger && a.partials),
+                 "sloppy hop: the norm-only mode has no dagger form and needs partials");
     a.one = 1.0f;
     WD_ARG_CHECK(ctx, wd_sloppy_supported(spinor_prec, gauge_prec, recon),
                  "sloppy hop: unsupported precision combination");

@@ -32,8 +32,8 @@
 //
 // Modes as for the other sloppy kernel: HOP_ONLY, HOP_M, HOP_M_DOT (per-system
 // FP64 partials of sum conj(dotv) out over the stored values), HOP_M_DOT_NORM
-// (plus |out|^2; no dagger form).  The partials are per wavefront, kLoopWaves
-// per workgroup ([rhs][nsb * kLoopWaves]), summed by shuffles without a
+// (plus |out|^2; no dagger form), HOP_M_NORM (|out|^2 only; no dagger form).
+// The partials are per wavefront, kLoopWaves per workgroup ([rhs][nsb * kLoopWaves]), summed by shuffles without a
 // workgroup barrier per system.
 #include "wd_fields.h"
 #include "wd_internal.h"
@@ -420,7 +420,7 @@ __device__ __forceinline__ void loop_block(const SloppyArgs &a, const Geom &g, i
             }
             unsigned q[12];
             const float sq = store16(a.out + off, a.out_scale + soff, stride, ui, acc, q);
-            if constexpr (MODE == HOP_M_DOT_NORM) {
+            if constexpr (MODE == HOP_M_DOT_NORM || MODE == HOP_M_NORM) {
                 float nq = 0.f;
 #pragma unroll
                 for (int k = 0; k < 12; ++k) {
@@ -453,6 +453,8 @@ __device__ __forceinline__ void loop_block(const SloppyArgs &a, const Geom &g, i
             wave_sum_store(pr, pi, a.partials + pidx);
         if constexpr (MODE == HOP_M_DOT_NORM)
             wave_sum_store(nn, 0.0, a.partials2 + pidx);
+        if constexpr (MODE == HOP_M_NORM)
+            wave_sum_store(nn, 0.0, a.partials + pidx);
     }
 }
 
@@ -540,6 +542,12 @@ void launch_mode(wd_ctx *ctx, const SloppyArgs &a, const Geom &g, int cb_first, 
                                dim3(kBlock), 0, ctx->stream, a, g, cb_first, nparity,
                                ctx->xcd_remap, per, ctx->parity_il);
         break;
+    case HOP_M_NORM:
+        if constexpr (!DAG)
+            hipLaunchKernelGGL((k_hop_sloppy_loop<false, HOP_M_NORM, K, LF>), dim3(grid),
+                               dim3(kBlock), 0, ctx->stream, a, g, cb_first, nparity,
+                               ctx->xcd_remap, per, ctx->parity_il);
+        break;
     default:
         hipLaunchKernelGGL((k_hop_sloppy_loop<DAG, HOP_M_DOT, K, LF>), dim3(grid), dim3(kBlock), 0,
                            ctx->stream, a, g, cb_first, nparity, ctx->xcd_remap, per, ctx->parity_il);
@@ -584,7 +592,8 @@ int wd_launch_hop_sloppy_loop(wd_ctx *ctx, const SloppyArgs &a, const Geom &g, i
     // automatic prefetch: 1 ahead at 3 wavefronts per SIMD for the hop and M
     // launches; 3 ahead at 2 for the dot launches, which spill 20-23
     // registers at 3 wavefronts per SIMD
-    const bool dot = mode == HOP_M_DOT || mode == HOP_M_DOT_NORM;
+    const bool dot = mode == HOP_M_DOT || mode == HOP_M_DOT_NORM; // HOP_M_NORM fits at 3
+
     const int K = ctx->loop_prefetch ? ctx->loop_prefetch : dot ? 3 : 1;
     if (K == 3) {
         if (dagger)

@@ -215,4 +215,5 @@ enum HopMode {
     HOP_M = 1,        // out[p] = a * same[p] - 1/2 hop(in)  (same == in for M)
     HOP_M_DOT = 2,    // as HOP_M, plus per-block partial of sum conj(dotv) * out
     HOP_M_DOT_NORM = 3, // as HOP_M_DOT, plus per-block partial of |out|^2 (sloppy kernel)
+    HOP_M_NORM = 4,   // as HOP_M, plus per-block partial (|out|^2, 0) (sloppy kernels, no dagger)
 };
