@@ -292,6 +292,24 @@ struct LoopShape<1, LL_F32> {
     static constexpr int waves = 2;
 };
 
+// Bit l of the result: system base + l is applied (active == 1); one flag per
+// lane, so every lane of the wavefront must be active.
+__device__ __forceinline__ unsigned long long applied_mask(const int *__restrict__ active, int base,
+                                                           int nrhs)
+{
+    const int r = base + (int)(threadIdx.x & 63);
+    return __ballot(r < nrhs && active[r] == 1);
+}
+
+// Index of the j-th (from 0) set bit of m; j < popcount(m).
+__device__ __forceinline__ int nth_set_bit(unsigned long long m, int j)
+{
+    const int lane = (int)(threadIdx.x & 63);
+    const int rank = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                    __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+    return __ffsll((long long)__ballot(((m >> lane) & 1ull) && rank == j)) - 1;
+}
+
 // One site block (kBlock sites of one parity) for systems [r0, r1).
 template <bool DAG, int MODE, int K, int LF>
 __device__ __forceinline__ void loop_block(const SloppyArgs &a, const Geom &g, int sb, int cb_first,
@@ -366,10 +384,23 @@ __device__ __forceinline__ void loop_block(const SloppyArgs &a, const Geom &g, i
         }
     }
 
+    // the applied systems of [r0, r1) as bits of a wavefront-uniform mask, 64
+    // systems at a time: a flag read per system would be a vector load whose
+    // wait (the loop's first) also drains the previous system's stores
+    unsigned long long am = ~0ull;
+    int abase = r0;
+    if (a.active)
+        am = applied_mask(a.active, r0, a.nrhs);
 #pragma unroll 1
     for (int r = r0; r < r1; ++r) {
-        if (a.active && a.active[r] != 1)
-            continue; // uniform over the launch
+        if (a.active) {
+            if (r - abase >= 64) {
+                abase = r;
+                am = applied_mask(a.active, r, a.nrhs);
+            }
+            if (!((am >> (r - abase)) & 1ull))
+                continue; // uniform over the launch
+        }
         cpx<float> acc[4][3];
 #pragma unroll
         for (int s = 0; s < 4; ++s)
@@ -456,24 +487,6 @@ __device__ __forceinline__ void loop_block(const SloppyArgs &a, const Geom &g, i
         if constexpr (MODE == HOP_M_NORM)
             wave_sum_store(nn, 0.0, a.partials + pidx);
     }
-}
-
-// Bit l of the result: system base + l is applied (active == 1); one flag per
-// lane, so every lane of the wavefront must be active.
-__device__ __forceinline__ unsigned long long applied_mask(const int *__restrict__ active, int base,
-                                                           int nrhs)
-{
-    const int r = base + (int)(threadIdx.x & 63);
-    return __ballot(r < nrhs && active[r] == 1);
-}
-
-// Index of the j-th (from 0) set bit of m; j < popcount(m).
-__device__ __forceinline__ int nth_set_bit(unsigned long long m, int j)
-{
-    const int lane = (int)(threadIdx.x & 63);
-    const int rank = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
-                                                    __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-    return __ffsll((long long)__ballot(((m >> lane) & 1ull) && rank == j)) - 1;
 }
 
 template <bool DAG, int MODE, int K, int LF>
