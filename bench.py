@@ -16,6 +16,8 @@ The JSON line carries
                  rocprofv3 FETCH_SIZE/WRITE_SIZE summary when present
   hop_roofline   one-parity FP64 hop (the reference's DslashoffdNew):
                  1536 B and 1320 flop per output site
+  fp64_recon12   the FP64 M and hop on 12-real FP64 links (SU(3) links,
+                 row 2 rebuilt in registers), beside the 18-real step
   sloppy_roofline  FP32-arithmetic M on F16 spinors and F32 12-real links:
                  8*48 + 3*52 = 540 B per site
   cg             time to a 1e-10 relative residual of M^dag M x = M^dag b:
@@ -811,6 +813,12 @@ def main():
         ms_m = ctx.time_op(w.OP_M, fout, fin, U, mass, False, 0, iters)
         tstream = ctx.get_option(w.OPT_TSTREAM)
         ms_hop = ctx.time_op(w.OP_HOP, fout, fin, U, mass, False, 0, iters)
+        # the same kernels on 12-real FP64 links (SU(3) links only; not the
+        # reference's format, so not the step)
+        U12 = ctx.gauge(A, prec=w.F64, recon=12)
+        ms_m12 = ctx.time_op(w.OP_M, fout, fin, U12, mass, False, 0, iters)
+        ms_hop12 = ctx.time_op(w.OP_HOP, fout, fin, U12, mass, False, 0, iters)
+        del U12
         # sloppy operator (the mixed solver's inner kernel), one system
         Us = ctx.gauge(A, prec=w.F32, recon=12)
         s_in = ctx.field(prec=w.F16, data=b)
@@ -863,6 +871,12 @@ def main():
                          "unit": "GB/s", "algorithmic_bytes_per_site": BYTES_HOP_F64,
                          "sites_per_launch": V // 2, "ms_per_launch": round(ms_hop, 4),
                          "gflops": round(FLOP_HOP * (V // 2) / (ms_hop * 1e-3) / 1e9, 1)},
+        "fp64_recon12": {"links": "F64 12-real (row 2 rebuilt; SU(3) links only)",
+                         "M_ms_per_launch": round(ms_m12, 4),
+                         "M_gflops": round(FLOP_M * V / (ms_m12 * 1e-3) / 1e9, 1),
+                         "M_algorithmic_bytes_per_site": 8 * 96 + 3 * 192,
+                         "hop_ms_per_launch": round(ms_hop12, 4),
+                         "hop_gflops": round(FLOP_HOP * (V // 2) / (ms_hop12 * 1e-3) / 1e9, 1)},
         "sloppy_roofline": {"kernel": "k_hop_sloppy<F16 spinors, F32 12-real links, HOP_M>",
                             "rhs_per_thread": best_R,
                             "achieved": round(BYTES_M_SLOPPY * V / (ms_s[best_R] * 1e-3) / 1e9, 1),

@@ -314,8 +314,7 @@ extern "C" int wd_cg(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd_gauge
     WD_ARG_CHECK(ctx, x != b, "cg: x and b must be distinct fields");
     WD_ARG_CHECK(ctx, x->prec == WD_F64 && b->prec == WD_F64 && x->nrhs == 1 && b->nrhs == 1,
                  "cg: single-rhs F64 fields required");
-    WD_ARG_CHECK(ctx, U->prec == WD_F64 && U->recon == WD_RECON_18,
-                 "cg: F64 18-real gauge required");
+    WD_ARG_CHECK(ctx, U->prec == WD_F64, "cg: F64 gauge required");
     WD_ARG_CHECK(ctx, params->max_iter >= 1, "cg: max_iter < 1");
     WD_ARG_CHECK(ctx, params->check_every >= 1, "cg: check_every < 1");
     WD_ARG_CHECK(ctx, params->tol >= 0.0, "cg: negative tolerance");
@@ -335,7 +334,7 @@ extern "C" int wd_cg(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd_gauge
     CGState *st = (CGState *)ctx->dscalars;
     double2 *d_scalar = (double2 *)((char *)ctx->dscalars + 256);
     double2 *partials = (double2 *)ctx->partials;
-    const double2 *Ud = (const double2 *)U->data;
+    const F64Links Ud = f64_links(U);
     double2 *xd = (double2 *)x->data;
     const double2 *bd = (const double2 *)b->data;
     long applies = 0;
@@ -482,8 +481,7 @@ extern "C" int wd_cg_eo(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd_ga
     WD_ARG_CHECK(ctx, x != b, "cg_eo: x and b must be distinct fields");
     WD_ARG_CHECK(ctx, x->prec == WD_F64 && b->prec == WD_F64 && x->nrhs == 1 && b->nrhs == 1,
                  "cg_eo: single-rhs F64 fields required");
-    WD_ARG_CHECK(ctx, U->prec == WD_F64 && U->recon == WD_RECON_18,
-                 "cg_eo: F64 18-real gauge required");
+    WD_ARG_CHECK(ctx, U->prec == WD_F64, "cg_eo: F64 gauge required");
     WD_ARG_CHECK(ctx, params->max_iter >= 1, "cg_eo: max_iter < 1");
     WD_ARG_CHECK(ctx, params->check_every >= 1, "cg_eo: check_every < 1");
     WD_ARG_CHECK(ctx, params->tol >= 0.0, "cg_eo: negative tolerance");
@@ -506,7 +504,7 @@ extern "C" int wd_cg_eo(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd_ga
     CGState *st = (CGState *)ctx->dscalars;
     double2 *d_scalar = (double2 *)((char *)ctx->dscalars + 256);
     double2 *partials = (double2 *)ctx->partials;
-    const double2 *Ud = (const double2 *)U->data;
+    const F64Links Ud = f64_links(U);
     double2 *xd = (double2 *)x->data;
     const double2 *bd = (const double2 *)b->data;
     long applies = 0;

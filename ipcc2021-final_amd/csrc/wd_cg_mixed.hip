@@ -1029,8 +1029,7 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
     WD_ARG_CHECK(ctx, x != b, "mixed cg: x and b must be distinct fields");
     WD_ARG_CHECK(ctx, x->prec == WD_F64 && b->prec == WD_F64, "mixed cg: x and b must be F64");
     WD_ARG_CHECK(ctx, x->nrhs == b->nrhs, "mixed cg: x and b nrhs differ");
-    WD_ARG_CHECK(ctx, U->prec == WD_F64 && U->recon == WD_RECON_18,
-                 "mixed cg: U must be F64 18-real (reliable updates)");
+    WD_ARG_CHECK(ctx, U->prec == WD_F64, "mixed cg: U must be F64 (reliable updates)");
     WD_ARG_CHECK(ctx, prm->spinor_prec == WD_F16 || prm->spinor_prec == WD_F32,
                  "mixed cg: spinor_prec must be F16 or F32");
     WD_ARG_CHECK(ctx, wd_sloppy_supported(prm->spinor_prec, Us->prec, Us->recon),
@@ -1102,7 +1101,7 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
                                          hipMemcpyHostToDevice, ctx->stream));
         WD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     }
-    const double2 *Ud = (const double2 *)U->data;
+    const F64Links Ud = f64_links(U);
     const size_t xe = x->rhs_elems;
     const size_t srb = rs.f->rhs_elems * prec_bytes_complex(sp);
     const size_t xsb = xs.f->rhs_elems * prec_bytes_complex(WD_F32);
@@ -1465,8 +1464,7 @@ extern "C" int wd_bicgstab_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, co
     WD_ARG_CHECK(ctx, x != b, "mixed bicgstab: x and b must be distinct fields");
     WD_ARG_CHECK(ctx, x->prec == WD_F64 && b->prec == WD_F64, "mixed bicgstab: x and b must be F64");
     WD_ARG_CHECK(ctx, x->nrhs == b->nrhs, "mixed bicgstab: x and b nrhs differ");
-    WD_ARG_CHECK(ctx, U->prec == WD_F64 && U->recon == WD_RECON_18,
-                 "mixed bicgstab: U must be F64 18-real (reliable updates)");
+    WD_ARG_CHECK(ctx, U->prec == WD_F64, "mixed bicgstab: U must be F64 (reliable updates)");
     WD_ARG_CHECK(ctx, prm->spinor_prec == WD_F16 || prm->spinor_prec == WD_F32,
                  "mixed bicgstab: spinor_prec must be F16 or F32");
     WD_ARG_CHECK(ctx, wd_sloppy_supported(prm->spinor_prec, Us->prec, Us->recon),
@@ -1534,7 +1532,7 @@ extern "C" int wd_bicgstab_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, co
                                          hipMemcpyHostToDevice, ctx->stream));
         WD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     }
-    const double2 *Ud = (const double2 *)U->data;
+    const F64Links Ud = f64_links(U);
     const size_t xe = x->rhs_elems;
     const size_t srb = r.f->rhs_elems * prec_bytes_complex(sp);
     const size_t xsb = xs.f->rhs_elems * prec_bytes_complex(WD_F32);

@@ -430,8 +430,7 @@ static int check_f64_op(wd_ctx *ctx, const wd_field *out, const wd_field *in, co
     WD_ARG_CHECK(ctx, out->nrhs == 1 && in->nrhs == 1, "operator: single-rhs fields required");
     if (U) {
         WD_ARG_CHECK(ctx, U->ctx == ctx, "operator: gauge from another context");
-        WD_ARG_CHECK(ctx, U->prec == WD_F64 && U->recon == WD_RECON_18,
-                     "operator: F64 18-real gauge required");
+        WD_ARG_CHECK(ctx, U->prec == WD_F64, "operator: F64 gauge required");
     }
     return WD_OK;
 }
@@ -452,7 +451,7 @@ extern "C" int wd_hop(wd_ctx *ctx, wd_field *out, const wd_field *in, const wd_g
                         sizeof(double2) * 12 * ctx->stride);
     if (rc)
         return rc;
-    rc = wd_launch_hop_f64(ctx, o, (const double2 *)in->data, (const double2 *)U->data, dagger,
+    rc = wd_launch_hop_f64(ctx, o, (const double2 *)in->data, f64_links(U), dagger,
                            cb, HOP_ONLY, 0.0, nullptr, nullptr, nullptr, nullptr);
     if (rc)
         return rc;
@@ -485,7 +484,7 @@ extern "C" int wd_apply_M(wd_ctx *ctx, wd_field *out, const wd_field *in, const 
     if (rc)
         return rc;
     rc = wd_launch_hop_f64(ctx, (double2 *)out->data, (const double2 *)in->data,
-                           (const double2 *)U->data, dagger, -1, HOP_M, 4.0 + mass,
+                           f64_links(U), dagger, -1, HOP_M, 4.0 + mass,
                            (const double2 *)in->data, nullptr, nullptr, nullptr);
     if (rc)
         return rc;
@@ -506,11 +505,11 @@ extern "C" int wd_apply_MdagM(wd_ctx *ctx, wd_field *out, const wd_field *in, wd
         return rc;
     WD_ARG_CHECK(ctx, tmp != out, "apply_MdagM: tmp and out must be distinct");
     rc = wd_launch_hop_f64(ctx, (double2 *)tmp->data, (const double2 *)in->data,
-                           (const double2 *)U->data, 0, -1, HOP_M, 4.0 + mass,
+                           f64_links(U), 0, -1, HOP_M, 4.0 + mass,
                            (const double2 *)in->data, nullptr, nullptr, nullptr);
     if (!rc)
         rc = wd_launch_hop_f64(ctx, (double2 *)out->data, (const double2 *)tmp->data,
-                               (const double2 *)U->data, 1, -1, HOP_M, 4.0 + mass,
+                               f64_links(U), 1, -1, HOP_M, 4.0 + mass,
                                (const double2 *)tmp->data, nullptr, nullptr, nullptr);
     if (rc)
         return rc;
@@ -575,11 +574,11 @@ extern "C" int wd_time_op(wd_ctx *ctx, int op, wd_field *out, const wd_field *in
     for (int k = 0; k < iters && !rc; ++k) {
         if (op == WD_OP_HOP)
             rc = wd_launch_hop_f64(ctx, (double2 *)out->data, (const double2 *)in->data,
-                                   (const double2 *)U->data, dagger, cb, HOP_ONLY, 0.0, nullptr,
+                                   f64_links(U), dagger, cb, HOP_ONLY, 0.0, nullptr,
                                    nullptr, nullptr, nullptr);
         else
             rc = wd_launch_hop_f64(ctx, (double2 *)out->data, (const double2 *)in->data,
-                                   (const double2 *)U->data, dagger, -1, HOP_M, 4.0 + mass,
+                                   f64_links(U), dagger, -1, HOP_M, 4.0 + mass,
                                    (const double2 *)in->data, nullptr, nullptr, nullptr);
     }
     hipError_t er = hipEventRecord(e1, ctx->stream);

@@ -56,17 +56,19 @@ __device__ __forceinline__ void load_spinor(const double2 *__restrict__ base, in
 }
 
 // NT: non-temporal (each link read once per launch); temporal when the two
-// parities of a site range run side by side and share links through L2
-template <bool NT>
+// parities of a site range run side by side and share links through L2.
+// LN: complex numbers stored per link, 9 (18-real, the reference's format) or
+// 6 (12-real: rows 0 and 1, row 2 rebuilt as conj(row0 x row1), SU(3) links).
+template <int LN, bool NT>
 __device__ __forceinline__ void load_link(const double2 *__restrict__ base, int stride, int j,
                                           cpx<double> (&U)[3][3])
 {
 #pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int b = 0; b < 3; ++b)
-            U[a][b] = NT ? ld_nt(base, f64_idx(a * 3 + b, (unsigned)j, 9))
-                         : ld(base, f64_idx(a * 3 + b, (unsigned)j, 9));
+    for (int k = 0; k < LN; ++k)
+        U[k / 3][k % 3] = NT ? ld_nt(base, f64_idx(k, (unsigned)j, LN))
+                             : ld(base, f64_idx(k, (unsigned)j, LN));
+    if constexpr (LN == 6)
+        reconstruct_row2(U);
 }
 
 // Forces the accumulator to be materialised here so the compiler cannot
@@ -81,7 +83,7 @@ __device__ __forceinline__ void pin_acc(cpx<double> (&acc)[4][3])
 }
 
 // One direction pair (forward and backward) of the hop.
-template <int MU, bool DAG, bool NTL>
+template <int MU, bool DAG, bool NTL, int LN>
 __device__ __forceinline__ void hop_mu(cpx<double> (&acc)[4][3], const double2 *__restrict__ in_o,
                                        const double2 *__restrict__ U, const Geom &g, int p, int i,
                                        int jf, int jb, double sf, double sb)
@@ -90,7 +92,7 @@ __device__ __forceinline__ void hop_mu(cpx<double> (&acc)[4][3], const double2 *
     {
         cpx<double> psi[4][3], W[3][3], h[2][3], r[2][3];
         load_spinor(in_o, stride, jf, psi);
-        load_link<NTL>(U + (long)((MU * 2 + p) * 9) * stride, stride, i, W);
+        load_link<LN, NTL>(U + (long)((MU * 2 + p) * LN) * stride, stride, i, W);
         using D = Dir<MU, true, DAG>;
         project<D>(psi, h);
         su3_mul_half<false>(W, h, r);
@@ -100,7 +102,7 @@ __device__ __forceinline__ void hop_mu(cpx<double> (&acc)[4][3], const double2 *
     {
         cpx<double> psi[4][3], W[3][3], h[2][3], r[2][3];
         load_spinor(in_o, stride, jb, psi);
-        load_link<NTL>(U + (long)((MU * 2 + (1 - p)) * 9) * stride, stride, jb, W);
+        load_link<LN, NTL>(U + (long)((MU * 2 + (1 - p)) * LN) * stride, stride, jb, W);
         using D = Dir<MU, false, DAG>;
         project<D>(psi, h);
         su3_mul_half<true>(W, h, r);
@@ -113,7 +115,7 @@ __device__ __forceinline__ void hop_mu(cpx<double> (&acc)[4][3], const double2 *
 // logical workgroups, so a link is read for its forward use by one and for
 // its backward use by the other shortly after (temporal link loads); else the
 // first half of the workgroups computes parity 0 and the second parity 1.
-template <bool DAG, int MODE, bool IL>
+template <bool DAG, int MODE, bool IL, int LN>
 // waves_per_eu(1, 1) lets the scheduler keep more of a direction's loads in
 // flight (~150 VGPRs, still 3 waves per SIMD at run time); measured faster
 // than the default budget on MI355X.
@@ -187,28 +189,28 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 1))) 
 
         {
             const int xf = (x + 1 == Lx) ? 0 : x + 1, xb = (x == 0) ? Lx - 1 : x - 1;
-            hop_mu<0, DAG, !IL>(acc, in_o, U, g, p, i, row + (xf >> 1), row + (xb >> 1),
+            hop_mu<0, DAG, !IL, LN>(acc, in_o, U, g, p, i, row + (xf >> 1), row + (xb >> 1),
                            x == Lx - 1 ? g.bc[0] : 1.0, x == 0 ? g.bc[0] : 1.0);
         }
         {
             const int s = g.Lxh;
             const int jf = (y + 1 == g.Ly) ? i - (g.Ly - 1) * s : i + s;
             const int jb = (y == 0) ? i + (g.Ly - 1) * s : i - s;
-            hop_mu<1, DAG, !IL>(acc, in_o, U, g, p, i, jf, jb, y == g.Ly - 1 ? g.bc[1] : 1.0,
+            hop_mu<1, DAG, !IL, LN>(acc, in_o, U, g, p, i, jf, jb, y == g.Ly - 1 ? g.bc[1] : 1.0,
                            y == 0 ? g.bc[1] : 1.0);
         }
         {
             const int s = g.Lxh * g.Ly;
             const int jf = (z + 1 == g.Lz) ? i - (g.Lz - 1) * s : i + s;
             const int jb = (z == 0) ? i + (g.Lz - 1) * s : i - s;
-            hop_mu<2, DAG, !IL>(acc, in_o, U, g, p, i, jf, jb, z == g.Lz - 1 ? g.bc[2] : 1.0,
+            hop_mu<2, DAG, !IL, LN>(acc, in_o, U, g, p, i, jf, jb, z == g.Lz - 1 ? g.bc[2] : 1.0,
                            z == 0 ? g.bc[2] : 1.0);
         }
         {
             const int s = g.Lxh * g.Ly * g.Lz;
             const int jf = (t + 1 == g.Lt) ? i - (g.Lt - 1) * s : i + s;
             const int jb = (t == 0) ? i + (g.Lt - 1) * s : i - s;
-            hop_mu<3, DAG, !IL>(acc, in_o, U, g, p, i, jf, jb, t == g.Lt - 1 ? g.bc[3] : 1.0,
+            hop_mu<3, DAG, !IL, LN>(acc, in_o, U, g, p, i, jf, jb, t == g.Lt - 1 ? g.bc[3] : 1.0,
                            t == 0 ? g.bc[3] : 1.0);
         }
 
@@ -279,7 +281,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 1))) 
 // applied to both; per system the operations are those of hop_mu, in the
 // same order, so each system's result is bit-identical to k_hop_f64's.
 // use0 / use1 are uniform over the workgroup.
-template <int MU, bool DAG, bool NTL>
+template <int MU, bool DAG, bool NTL, int LN>
 __device__ __forceinline__ void hop_mu2(cpx<double> (&acc0)[4][3], cpx<double> (&acc1)[4][3],
                                         bool use0, bool use1,
                                         const double2 *__restrict__ in0,
@@ -290,7 +292,7 @@ __device__ __forceinline__ void hop_mu2(cpx<double> (&acc0)[4][3], cpx<double> (
     const int stride = g.stride;
     {
         cpx<double> W[3][3];
-        load_link<NTL>(U + (long)((MU * 2 + p) * 9) * stride, stride, i, W);
+        load_link<LN, NTL>(U + (long)((MU * 2 + p) * LN) * stride, stride, i, W);
         using D = Dir<MU, true, DAG>;
         if (use0) {
             cpx<double> psi[4][3], h[2][3], r[2][3];
@@ -311,7 +313,7 @@ __device__ __forceinline__ void hop_mu2(cpx<double> (&acc0)[4][3], cpx<double> (
     pin_acc(acc1);
     {
         cpx<double> W[3][3];
-        load_link<NTL>(U + (long)((MU * 2 + (1 - p)) * 9) * stride, stride, jb, W);
+        load_link<LN, NTL>(U + (long)((MU * 2 + (1 - p)) * LN) * stride, stride, jb, W);
         using D = Dir<MU, false, DAG>;
         if (use0) {
             cpx<double> psi[4][3], h[2][3], r[2][3];
@@ -355,7 +357,7 @@ __device__ __forceinline__ void store_out(double2 *__restrict__ out_p,
 // 2 blockIdx.y + 1): the link traffic, three quarters of an FP64 hop's bytes,
 // is paid once per pair.  HOP_ONLY and HOP_M; same work order, masks and
 // per-system coefficients as k_hop_f64 with a batch.
-template <bool DAG, int MODE, bool IL>
+template <bool DAG, int MODE, bool IL, int LN>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_hop_f64_pair(
     double2 *__restrict__ out, const double2 *__restrict__ in, const double2 *__restrict__ U,
     Geom g, int cb_first, int nparity, double a, double hc, const double2 *__restrict__ same,
@@ -409,28 +411,28 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 1))) 
             acc0[s][c] = acc1[s][c] = {0.0, 0.0};
     {
         const int xf = (x + 1 == Lx) ? 0 : x + 1, xb = (x == 0) ? Lx - 1 : x - 1;
-        hop_mu2<0, DAG, !IL>(acc0, acc1, use0, use1, in0, in1, U, g, p, i, row + (xf >> 1),
+        hop_mu2<0, DAG, !IL, LN>(acc0, acc1, use0, use1, in0, in1, U, g, p, i, row + (xf >> 1),
                              row + (xb >> 1), x == Lx - 1 ? g.bc[0] : 1.0, x == 0 ? g.bc[0] : 1.0);
     }
     {
         const int s = g.Lxh;
         const int jf = (y + 1 == g.Ly) ? i - (g.Ly - 1) * s : i + s;
         const int jb = (y == 0) ? i + (g.Ly - 1) * s : i - s;
-        hop_mu2<1, DAG, !IL>(acc0, acc1, use0, use1, in0, in1, U, g, p, i, jf, jb,
+        hop_mu2<1, DAG, !IL, LN>(acc0, acc1, use0, use1, in0, in1, U, g, p, i, jf, jb,
                              y == g.Ly - 1 ? g.bc[1] : 1.0, y == 0 ? g.bc[1] : 1.0);
     }
     {
         const int s = g.Lxh * g.Ly;
         const int jf = (z + 1 == g.Lz) ? i - (g.Lz - 1) * s : i + s;
         const int jb = (z == 0) ? i + (g.Lz - 1) * s : i - s;
-        hop_mu2<2, DAG, !IL>(acc0, acc1, use0, use1, in0, in1, U, g, p, i, jf, jb,
+        hop_mu2<2, DAG, !IL, LN>(acc0, acc1, use0, use1, in0, in1, U, g, p, i, jf, jb,
                              z == g.Lz - 1 ? g.bc[2] : 1.0, z == 0 ? g.bc[2] : 1.0);
     }
     {
         const int s = g.Lxh * g.Ly * g.Lz;
         const int jf = (t + 1 == g.Lt) ? i - (g.Lt - 1) * s : i + s;
         const int jb = (t == 0) ? i + (g.Lt - 1) * s : i - s;
-        hop_mu2<3, DAG, !IL>(acc0, acc1, use0, use1, in0, in1, U, g, p, i, jf, jb,
+        hop_mu2<3, DAG, !IL, LN>(acc0, acc1, use0, use1, in0, in1, U, g, p, i, jf, jb,
                              t == g.Lt - 1 ? g.bc[3] : 1.0, t == 0 ? g.bc[3] : 1.0);
     }
     const long oo = (long)p * 12 * g.stride;
@@ -456,7 +458,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 1))) 
 // order of k_hop_f64 (x, y, z, t forward, t backward), so out is bit-identical
 // to k_hop_f64's.  HOP_M_DOT partials are per workgroup, indexed by the work
 // unit (t run, column block, type).
-template <bool DAG, int MODE>
+template <bool DAG, int MODE, int LN>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_hop_f64_ts(
     double2 *__restrict__ out, const double2 *__restrict__ in, const double2 *__restrict__ U,
     Geom g, double a, double hc, const double2 *__restrict__ same,
@@ -504,7 +506,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 1))) 
         const int ib = ((tb * g.Lz + z) * g.Ly + y) * g.Lxh + xh;
         cpx<double> psi[4][3], W[3][3], h[2][3];
         load_spinor(in + (long)pb * 12 * stride, stride, ib, psi);
-        load_link<true>(U + (long)((3 * 2 + pb) * 9) * stride, stride, ib, W);
+        load_link<LN, true>(U + (long)((3 * 2 + pb) * LN) * stride, stride, ib, W);
         using D = Dir<3, false, DAG>;
         project<D>(psi, h);
         su3_mul_half<true>(W, h, rc);
@@ -528,15 +530,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 1))) 
 #pragma unroll
             for (int cc = 0; cc < 3; ++cc)
                 acc[s][cc] = {0.0, 0.0};
-        hop_mu<0, DAG, false>(acc, in_o, U, g, p, i, row + (xf >> 1), row + (xb >> 1), sxf, sxb);
-        hop_mu<1, DAG, false>(acc, in_o, U, g, p, i, i + dyf, i + dyb, syf, syb);
-        hop_mu<2, DAG, false>(acc, in_o, U, g, p, i, i + dzf, i + dzb, szf, szb);
+        hop_mu<0, DAG, false, LN>(acc, in_o, U, g, p, i, row + (xf >> 1), row + (xb >> 1), sxf, sxb);
+        hop_mu<1, DAG, false, LN>(acc, in_o, U, g, p, i, i + dyf, i + dyb, syf, syb);
+        hop_mu<2, DAG, false, LN>(acc, in_o, U, g, p, i, i + dzf, i + dzb, szf, szb);
         // forward t with the site's t link, kept for the next site's backward
         // hop; no other thread reads it, so it loads non-temporal and leaves
         // L2 to the spinors and x, y, z links the neighbouring columns share
         // (32^3x64 M 0.447 -> 0.437 ms on MI355X)
         cpx<double> W[3][3];
-        load_link<true>(U + (long)((3 * 2 + p) * 9) * stride, stride, i, W);
+        load_link<LN, true>(U + (long)((3 * 2 + p) * LN) * stride, stride, i, W);
         {
             const int jf = (t + 1 == g.Lt) ? i - (g.Lt - 1) * slice : i + slice;
             cpx<double> psi[4][3], h[2][3], r[2][3];
@@ -627,7 +629,7 @@ static int tstream_grid(const Geom &g, int tb)
     return 2 * ncb * (g.Lt / tb);
 }
 
-template <bool DAG>
+template <bool DAG, int LN>
 int launch_mode(wd_ctx *ctx, double2 *out, const double2 *in, const double2 *U, int cb, int mode,
                 double a, double hc, const double2 *same, const double2 *dotv, double2 *partials,
                 const int *stop, const F64Batch &b, int nrhs)
@@ -648,7 +650,7 @@ int launch_mode(wd_ctx *ctx, double2 *out, const double2 *in, const double2 *U, 
         if (mode == HOP_M_DOT && partials == ctx->partials && gts > ctx->max_partials)
             return wd_set_error(ctx, WD_EARG, "hop: too many workgroups for the partials buffer");
 #define WD_HOP_TS_LAUNCH(M)                                                                        \
-    hipLaunchKernelGGL((k_hop_f64_ts<DAG, M>), dim3(glaunch), dim3(kBlock), 0, ctx->stream, out, in, \
+    hipLaunchKernelGGL((k_hop_f64_ts<DAG, M, LN>), dim3(glaunch), dim3(kBlock), 0, ctx->stream, out, in, \
                        U, g, a, hc, same, dotv, partials, tb, ncb, gts, ctx->xcd_remap, stop)
         if (mode == HOP_M)
             WD_HOP_TS_LAUNCH(HOP_M);
@@ -661,7 +663,7 @@ int launch_mode(wd_ctx *ctx, double2 *out, const double2 *in, const double2 *U, 
     if (nrhs >= 2 && ctx->f64_pairs && (mode == HOP_ONLY || mode == HOP_M)) {
         const dim3 gp(grid, (nrhs + 1) / 2);
 #define WD_HOP_PAIR_LAUNCH(M, I)                                                                   \
-    hipLaunchKernelGGL((k_hop_f64_pair<DAG, M, I>), gp, dim3(kBlock), 0, ctx->stream, out, in, U, \
+    hipLaunchKernelGGL((k_hop_f64_pair<DAG, M, I, LN>), gp, dim3(kBlock), 0, ctx->stream, out, in, U, \
                        g, cb_first, nparity, a, hc, same, ctx->xcd_remap, stop, b, nrhs)
         if (mode == HOP_ONLY) {
             if (il)
@@ -679,7 +681,7 @@ int launch_mode(wd_ctx *ctx, double2 *out, const double2 *in, const double2 *U, 
     }
     const dim3 gd(grid, nrhs);
 #define WD_HOP_F64_LAUNCH(M, I)                                                                    \
-    hipLaunchKernelGGL((k_hop_f64<DAG, M, I>), gd, dim3(kBlock), 0, ctx->stream, out, in, U, g,  \
+    hipLaunchKernelGGL((k_hop_f64<DAG, M, I, LN>), gd, dim3(kBlock), 0, ctx->stream, out, in, U, g,  \
                        cb_first, nparity, a, hc, same, dotv, partials, ctx->xcd_remap, stop, b)
     switch (mode) {
     case HOP_ONLY:
@@ -705,24 +707,37 @@ int launch_mode(wd_ctx *ctx, double2 *out, const double2 *in, const double2 *U, 
     return wd_check_launch(ctx, "k_hop_f64");
 }
 
+// Dispatch on the link format: 18-real (the reference's) or 12-real links.
+template <bool DAG>
+int launch_links(wd_ctx *ctx, double2 *out, const double2 *in, F64Links U, int cb, int mode,
+                 double a, double hc, const double2 *same, const double2 *dotv,
+                 double2 *partials, const int *stop, const F64Batch &b, int nrhs)
+{
+    if (U.recon == WD_RECON_12)
+        return launch_mode<DAG, 6>(ctx, out, in, U.p, cb, mode, a, hc, same, dotv, partials, stop,
+                                   b, nrhs);
+    return launch_mode<DAG, 9>(ctx, out, in, U.p, cb, mode, a, hc, same, dotv, partials, stop, b,
+                               nrhs);
+}
+
 } // namespace
 
 // cb in {0, 1}: one output parity; cb = -1: both parities in one launch.
 // out = a same + hc hop(in) (HOP_ONLY: hc hop(in)); hc = -1/2 is the Wilson
 // operator's coefficient.
-int wd_launch_hop_f64c(wd_ctx *ctx, double2 *out, const double2 *in, const double2 *U, int dagger,
+int wd_launch_hop_f64c(wd_ctx *ctx, double2 *out, const double2 *in, F64Links U, int dagger,
                        int cb, int mode, double a, double hc, const double2 *same,
                        const double2 *dotv, double2 *partials, const int *stop)
 {
     const F64Batch one = {0, nullptr, nullptr, nullptr};
     if (dagger)
-        return launch_mode<true>(ctx, out, in, U, cb, mode, a, hc, same, dotv, partials, stop,
-                                 one, 1);
-    return launch_mode<false>(ctx, out, in, U, cb, mode, a, hc, same, dotv, partials, stop, one,
-                              1);
+        return launch_links<true>(ctx, out, in, U, cb, mode, a, hc, same, dotv, partials, stop,
+                                  one, 1);
+    return launch_links<false>(ctx, out, in, U, cb, mode, a, hc, same, dotv, partials, stop, one,
+                               1);
 }
 
-int wd_launch_hop_f64(wd_ctx *ctx, double2 *out, const double2 *in, const double2 *U, int dagger,
+int wd_launch_hop_f64(wd_ctx *ctx, double2 *out, const double2 *in, F64Links U, int dagger,
                       int cb, int mode, double a, const double2 *same, const double2 *dotv,
                       double2 *partials, const int *stop)
 {
@@ -733,19 +748,19 @@ int wd_launch_hop_f64(wd_ctx *ctx, double2 *out, const double2 *in, const double
 // nrhs systems at once (blockIdx.y = rhs); b.mask skips systems, b.a_rhs and
 // b.c_rhs give per-system diagonal and hop coefficients (else hc);
 // partials are [rhs][workgroup].
-int wd_launch_hop_f64_batch_c(wd_ctx *ctx, double2 *out, const double2 *in, const double2 *U,
+int wd_launch_hop_f64_batch_c(wd_ctx *ctx, double2 *out, const double2 *in, F64Links U,
                               int dagger, int cb, int mode, double hc, const double2 *same,
                               const double2 *dotv, double2 *partials, const int *stop,
                               const F64Batch &b, int nrhs)
 {
     if (dagger)
-        return launch_mode<true>(ctx, out, in, U, cb, mode, 0.0, hc, same, dotv, partials, stop,
-                                 b, nrhs);
-    return launch_mode<false>(ctx, out, in, U, cb, mode, 0.0, hc, same, dotv, partials, stop, b,
-                              nrhs);
+        return launch_links<true>(ctx, out, in, U, cb, mode, 0.0, hc, same, dotv, partials, stop,
+                                  b, nrhs);
+    return launch_links<false>(ctx, out, in, U, cb, mode, 0.0, hc, same, dotv, partials, stop, b,
+                               nrhs);
 }
 
-int wd_launch_hop_f64_batch(wd_ctx *ctx, double2 *out, const double2 *in, const double2 *U,
+int wd_launch_hop_f64_batch(wd_ctx *ctx, double2 *out, const double2 *in, F64Links U,
                             int dagger, int cb, int mode, const double2 *same,
                             const double2 *dotv, double2 *partials, const int *stop,
                             const F64Batch &b, int nrhs)

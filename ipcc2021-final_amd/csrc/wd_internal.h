@@ -169,7 +169,19 @@ inline size_t prec_bytes_complex(int prec)
 }
 
 // ---- internal entry points shared between translation units --------------------
-int wd_launch_hop_f64(wd_ctx *ctx, double2 *out, const double2 *in, const double2 *U, int dagger,
+// FP64 links of a hop launch: 18-real (9 complex per link, the reference's
+// format; what a bare pointer means) or 12-real (rows 0 and 1, row 2 rebuilt
+// as conj(row0 x row1): SU(3) links only).
+struct F64Links {
+    const double2 *p;
+    int recon;
+    F64Links(const double2 *q, int r = WD_RECON_18) : p(q), recon(r) {}
+};
+inline F64Links f64_links(const wd_gauge *U)
+{
+    return F64Links(static_cast<const double2 *>(U->data), U->recon);
+}
+int wd_launch_hop_f64(wd_ctx *ctx, double2 *out, const double2 *in, F64Links U, int dagger,
                       int cb, int mode, double a, const double2 *same, const double2 *dotv,
                       double2 *partials, const int *stop);
 int wd_launch_zero(wd_ctx *ctx, void *p, size_t bytes);
@@ -196,15 +208,15 @@ struct F64Batch {
     const double *a_rhs;
     const double *c_rhs;
 };
-int wd_launch_hop_f64_batch(wd_ctx *ctx, double2 *out, const double2 *in, const double2 *U,
+int wd_launch_hop_f64_batch(wd_ctx *ctx, double2 *out, const double2 *in, F64Links U,
                             int dagger, int cb, int mode, const double2 *same,
                             const double2 *dotv, double2 *partials, const int *stop,
                             const F64Batch &b, int nrhs);
 // general hop coefficient: out = a same + hc hop(in), hc = -1/2 for the Wilson M
-int wd_launch_hop_f64c(wd_ctx *ctx, double2 *out, const double2 *in, const double2 *U, int dagger,
+int wd_launch_hop_f64c(wd_ctx *ctx, double2 *out, const double2 *in, F64Links U, int dagger,
                        int cb, int mode, double a, double hc, const double2 *same,
                        const double2 *dotv, double2 *partials, const int *stop);
-int wd_launch_hop_f64_batch_c(wd_ctx *ctx, double2 *out, const double2 *in, const double2 *U,
+int wd_launch_hop_f64_batch_c(wd_ctx *ctx, double2 *out, const double2 *in, F64Links U,
                               int dagger, int cb, int mode, double hc, const double2 *same,
                               const double2 *dotv, double2 *partials, const int *stop,
                               const F64Batch &b, int nrhs);
