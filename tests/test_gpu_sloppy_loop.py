@@ -169,9 +169,17 @@ def test_loop_groups_and_prefetch_bit_identical(w):
         with pytest.raises(w.WilsonError) as e:
             ctx.set_option(w.OPT_LOOP_SYSTEMS, -1)
         assert e.value.code == w.WD_EARG
-        # the site-chunk argument of the loop kernel counts systems per pass
+        # parity-major work order (both parities of M)
         ctx.set_option(w.OPT_LOOP_PREFETCH, 0)
         ctx.set_option(w.OPT_LOOP_SYSTEMS, 0)
+        ctx.set_option(w.OPT_PARITY_ORDER, 0)
+        got = []
+        for dag in (False, True):
+            ctx.apply_M_sloppy(fout, fin, Us, masses, dag, w.RHS_LOOP)
+            got += [fout.download(r) for r in range(n)]
+        ctx.set_option(w.OPT_PARITY_ORDER, 1)
+        assert all(np.array_equal(a, b) for a, b in zip(got, runs[(0, 0)][0][:2 * n]))
+        # the site-chunk argument of the loop kernel counts systems per pass
         for chunk in (3, 16):
             got = []
             for dag in (False, True):
