@@ -203,7 +203,8 @@ extern "C" int wd_link_product(wd_ctx *ctx, int impl, int nsites, int nsys, cons
                     hipEventSynchronize(e1) != hipSuccess ||
                     hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
             rc = wd_set_error(ctx, WD_EHIP, "link_product: timing failed");
-        if (!rc && hipMemcpy(out, dout, ob, hipMemcpyDeviceToHost) != hipSuccess)
+        if (!rc && (hipMemcpyAsync(out, dout, ob, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+                    hipStreamSynchronize(ctx->stream) != hipSuccess))
             rc = wd_set_error(ctx, WD_EHIP, "link_product: copy back failed");
         if (!rc)
             *ms_per_launch = (double)ms / iters;
