@@ -248,8 +248,9 @@ __device__ __forceinline__ void wave_sum_store(double sr, double si, double2 *ds
 }
 
 // The 8 hop directions of one system, d = 2 mu + (backward), with the
-// neighbour loads K directions ahead in K + 1 rotating buffers.
-template <bool DAG, int MODE, int K, int LF>
+// neighbour loads K directions ahead in K + 1 rotating buffers; the loads of
+// directions 0 .. PRE - 1 (PRE = K or K + 1) were issued before the walk.
+template <bool DAG, int MODE, int K, int LF, int PRE>
 struct Walk {
     const SloppyArgs &a;
     const LinkRegs<LF> &L;
@@ -263,7 +264,7 @@ struct Walk {
     template <int D>
     __device__ __forceinline__ void step(cpx<float> (&acc)[4][3], Nbr (&nb)[K + 1]) const
     {
-        if constexpr (D + K < 8)
+        if constexpr (D + K < 8 && D + K >= PRE)
             load_nbr(inb, isc, j[D + K], stride, nb[(D + K) % (K + 1)]);
         else if constexpr (D + K == 8 && MODE != HOP_ONLY)
             load_nbr(a.same + off, a.same_scale + soff, ui, stride, nb[8 % (K + 1)]);
@@ -391,16 +392,59 @@ __device__ __forceinline__ void loop_block(const SloppyArgs &a, const Geom &g, i
     int abase = r0;
     if (a.active)
         am = applied_mask(a.active, r0, a.nrhs);
-#pragma unroll 1
-    for (int r = r0; r < r1; ++r) {
-        if (a.active) {
+    // the first applied system at or after r (r1 when there is none); uniform
+    // over the wavefront, so every lane takes part in the ballot
+    auto next_applied = [&](int r) -> int {
+        if (!a.active)
+            return r;
+        for (; r < r1; ++r) {
             if (r - abase >= 64) {
                 abase = r;
                 am = applied_mask(a.active, r, a.nrhs);
             }
-            if (!((am >> (r - abase)) & 1ull))
-                continue; // uniform over the launch
+            if ((am >> (r - abase)) & 1ull)
+                return r;
         }
+        return r1;
+    };
+    // the neighbour offsets are opaque per system: hoisted out of the loop,
+    // their 32 byte offsets (3 chunks + scale per neighbour) would stay live
+    // as 64-bit values
+    auto opaque_offsets = [&](unsigned (&j)[8]) {
+#pragma unroll
+        for (int d = 0; d < 8; ++d) {
+            j[d] = jn[d];
+            asm volatile("" : "+v"(j[d]));
+        }
+    };
+    const char *in_p = a.in + (size_t)(1 - p) * pbytes;
+    const float *isc_p = a.in_scale + (size_t)(1 - p) * stride;
+    const unsigned ui = (unsigned)i;
+
+    // Neighbour buffers: the loads of direction d + K are issued before the
+    // arithmetic of direction d, the site's own spinor (the diagonal term)
+    // after the last direction's loads.  The loads of the next system's first
+    // PRE directions are issued before this system's stores: loads and
+    // stores leave the memory counter in the order they were issued, so a
+    // wait for a load issued after the stores would also wait for the
+    // stores to complete.
+    // (two directions ahead where the registers allow; the |out|^2 launch
+    // would spill with two)
+    constexpr int PRE = (K == 1 && MODE != HOP_M_NORM) ? 2 : K;
+    Nbr nb[K + 1];
+    int r = next_applied(r0);
+    if (site_ok && r < r1) {
+        unsigned j[8];
+        opaque_offsets(j);
+        const char *inb = in_p + (size_t)r * a.rhs_bytes;
+        const float *isc = isc_p + (size_t)r * 2 * stride;
+#pragma unroll
+        for (int d = 0; d < PRE; ++d)
+            load_nbr(inb, isc, j[d], stride, nb[d]);
+    }
+#pragma unroll 1
+    while (r < r1) {
+        const int rn = next_applied(r + 1);
         cpx<float> acc[4][3];
 #pragma unroll
         for (int s = 0; s < 4; ++s)
@@ -411,26 +455,11 @@ __device__ __forceinline__ void loop_block(const SloppyArgs &a, const Geom &g, i
         const size_t soff = (size_t)r * 2 * stride + (size_t)p * stride;
         double pr = 0.0, pi = 0.0, nn = 0.0;
         if (site_ok) {
-            const char *inb = a.in + (size_t)r * a.rhs_bytes + (size_t)(1 - p) * pbytes;
-            const float *isc = a.in_scale + (size_t)r * 2 * stride + (size_t)(1 - p) * stride;
-            // the neighbour offsets are opaque per system: hoisted out of the
-            // loop, their 32 byte offsets (3 chunks + scale per neighbour)
-            // would stay live as 64-bit values
+            const char *inb = in_p + (size_t)r * a.rhs_bytes;
+            const float *isc = isc_p + (size_t)r * 2 * stride;
             unsigned j[8];
-#pragma unroll
-            for (int d = 0; d < 8; ++d) {
-                j[d] = jn[d];
-                asm volatile("" : "+v"(j[d]));
-            }
-            const unsigned ui = (unsigned)i;
-            // neighbour buffers: the loads of direction d + K are issued before
-            // the arithmetic of direction d; the site's own spinor (the
-            // diagonal term) follows the last direction's loads
-            Nbr nb[K + 1];
-            const Walk<DAG, MODE, K, LF> walk{a, L, j, inb, isc, off, soff, ui, sbits, stride};
-#pragma unroll
-            for (int d = 0; d < K; ++d)
-                load_nbr(inb, isc, j[d], stride, nb[d]);
+            opaque_offsets(j);
+            const Walk<DAG, MODE, K, LF, PRE> walk{a, L, j, inb, isc, off, soff, ui, sbits, stride};
             walk.template step<0>(acc, nb);
             walk.template step<1>(acc, nb);
             walk.template step<2>(acc, nb);
@@ -450,7 +479,23 @@ __device__ __forceinline__ void loop_block(const SloppyArgs &a, const Geom &g, i
                 }
             }
             unsigned q[12];
-            const float sq = store16(a.out + off, a.out_scale + soff, stride, ui, acc, q);
+            const float sq = pack16(acc, q);
+            // the dot product's operand, then the next system's first
+            // neighbours, are loaded before the stores
+            unsigned dv[12];
+            float sd = 0.f;
+            if constexpr (MODE == HOP_M_DOT || MODE == HOP_M_DOT_NORM) {
+                load16(a.dotv + off, stride, ui, dv);
+                sd = load_scale(a.dotv_scale + soff, ui);
+            }
+            if (rn < r1) {
+                const char *inb_n = in_p + (size_t)rn * a.rhs_bytes;
+                const float *isc_n = isc_p + (size_t)rn * 2 * stride;
+#pragma unroll
+                for (int d = 0; d < PRE; ++d)
+                    load_nbr(inb_n, isc_n, j[d], stride, nb[d]);
+            }
+            store16w(a.out + off, a.out_scale + soff, stride, ui, q, sq);
             if constexpr (MODE == HOP_M_DOT_NORM || MODE == HOP_M_NORM) {
                 float nq = 0.f;
 #pragma unroll
@@ -461,16 +506,13 @@ __device__ __forceinline__ void loop_block(const SloppyArgs &a, const Geom &g, i
                 nn = (double)nq * ((double)sq * (double)sq);
             }
             if constexpr (MODE == HOP_M_DOT || MODE == HOP_M_DOT_NORM) {
-                unsigned d[12];
-                load16(a.dotv + off, stride, ui, d);
-                const float sd = load_scale(a.dotv_scale + soff, ui);
                 float tr = 0.f, ti = 0.f;
 #pragma unroll
                 for (int k = 0; k < 12; ++k) {
-                    tr = fmaf(re16(d[k]), re16(q[k]), tr);
-                    tr = fmaf(im16(d[k]), im16(q[k]), tr);
-                    ti = fmaf(re16(d[k]), im16(q[k]), ti);
-                    ti = fmaf(-im16(d[k]), re16(q[k]), ti);
+                    tr = fmaf(re16(dv[k]), re16(q[k]), tr);
+                    tr = fmaf(im16(dv[k]), im16(q[k]), tr);
+                    ti = fmaf(re16(dv[k]), im16(q[k]), ti);
+                    ti = fmaf(-im16(dv[k]), re16(q[k]), ti);
                 }
                 const double ss = (double)sd * (double)sq;
                 pr = (double)tr * ss;
@@ -486,6 +528,7 @@ __device__ __forceinline__ void loop_block(const SloppyArgs &a, const Geom &g, i
             wave_sum_store(nn, 0.0, a.partials2 + pidx);
         if constexpr (MODE == HOP_M_NORM)
             wave_sum_store(nn, 0.0, a.partials + pidx);
+        r = rn;
     }
 }
 

@@ -160,26 +160,31 @@ __device__ __forceinline__ void block_sum_store(double sr, double si, double2 *d
     __syncthreads();
 }
 
-// Stores an F16 site (values / max |component|, scale = that max) and returns
-// the packed words and the scale as stored.
-__device__ __forceinline__ float store16(char *base, float *scale, int stride, unsigned i,
-                                         const cpx<float> (&v)[4][3], unsigned (&w)[12])
+// The F16 form of a site: words = values / max |component| (packed halves),
+// returns the scale (that max).  Below 1e-30 the site is stored as zero (1/m
+// would overflow for denormal m, turning zeros into NaN); NaN inputs still
+// propagate.
+__device__ __forceinline__ float pack16(const cpx<float> (&v)[4][3], unsigned (&w)[12])
 {
     float m = 0.f;
 #pragma unroll
     for (int k = 0; k < 12; ++k)
         m = fmaxf(m, fmaxf(fabsf(v[k / 3][k % 3].r), fabsf(v[k / 3][k % 3].i)));
-    // below 1e-30 the site is stored as zero (1/m would overflow for denormal
-    // m, turning zeros into NaN); NaN inputs still propagate
     const bool tiny = m < 1.0e-30f;
     const float inv = tiny ? 0.f : 1.0f / m;
-    const float s = tiny ? 0.f : m;
-    // outputs are streamed (non-temporal) so they do not evict from L2 the
-    // input spinors that neighbouring workgroups still read
-    __builtin_nontemporal_store(s, reinterpret_cast<float *>(reinterpret_cast<char *>(scale) + i * 4u));
 #pragma unroll
     for (int k = 0; k < 12; ++k)
         w[k] = f2h(v[k / 3][k % 3].r * inv, v[k / 3][k % 3].i * inv);
+    return tiny ? 0.f : m;
+}
+
+// Stores the F16 words and scale of site i.  Outputs are streamed
+// (non-temporal) so they do not evict from L2 the input spinors that
+// neighbouring workgroups still read.
+__device__ __forceinline__ void store16w(char *base, float *scale, int stride, unsigned i,
+                                         const unsigned (&w)[12], float s)
+{
+    __builtin_nontemporal_store(s, reinterpret_cast<float *>(reinterpret_cast<char *>(scale) + i * 4u));
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         u4v c;
@@ -189,6 +194,14 @@ __device__ __forceinline__ float store16(char *base, float *scale, int stride, u
         c.w = w[4 * q + 3];
         __builtin_nontemporal_store(c, reinterpret_cast<u4v *>(base + off16(q, stride, i)));
     }
+}
+
+// Stores an F16 site and returns the packed words and the scale as stored.
+__device__ __forceinline__ float store16(char *base, float *scale, int stride, unsigned i,
+                                         const cpx<float> (&v)[4][3], unsigned (&w)[12])
+{
+    const float s = pack16(v, w);
+    store16w(base, scale, stride, i, w, s);
     return s;
 }
 
