@@ -20,6 +20,9 @@ The JSON line carries
                  row 2 rebuilt in registers), beside the 18-real step
   sloppy_roofline  FP32-arithmetic M on F16 spinors and F32 12-real links:
                  8*48 + 3*52 = 540 B per site
+  sloppy_temporal_gauge  the same on the links in temporal gauge (the mixed
+                 solvers' links here): t links only on the boundary slices,
+                 6*48 + 2*48/Lt + 3*52 B per site
   cg             time to a 1e-10 relative residual of M^dag M x = M^dag b:
                  FP64 CG (the reference's algorithm) vs the mixed solver
   cpu_baseline   the reference's own DslashNew (compiled from its sources,
@@ -31,6 +34,7 @@ The JSON line carries
                  mixed solve against 12 FP64 CG solves one after another
   batched        24^3x48, 12 point sources x 4 kappas = 48 systems
   large          48^3x96, 12 Gaussian systems, F16 spinors and F16 links
+The mixed solvers run on sloppy links in temporal gauge (WD_RECON_12T).
 --workload single runs only the single-system part.
 """
 import argparse
@@ -53,6 +57,13 @@ FLOP_M = 1320 + 48         # per site, hop + (4+m) diagonal (fused FMA)
 BYTES_HOP_F64 = 8 * 144 + 2 * 192
 BYTES_M_F64 = 8 * 144 + 3 * 192
 BYTES_M_SLOPPY = 8 * 48 + 3 * 52   # F32 12-real links, F16 spinors + float scale
+# the same links in temporal gauge: the t links are loaded on the boundary
+# slices only (2 of 8 links on 1 of Lt slices each)
+def bytes_m_sloppy_tg(Lt):
+    return 6 * 48 + 2 * 48 / Lt + 3 * 52
+
+
+TG_LABEL = " in temporal gauge"
 PMC_SUMMARY = {0: os.path.join(ROOT, "profiles", "k_hop_f64_M_32x32x32x64_pmc.txt"),
                16: os.path.join(ROOT, "profiles", "k_hop_f64_ts_M_32x32x32x64_pmc.txt")}
 
@@ -242,11 +253,11 @@ def cg_compare(w, ctx, U, A, b, mass, max_iter=20000, variants=CG_VARIANTS):
                                 "solves_per_s": round(1.0 / fpe["seconds"], 3),
                                 "true_rel_residual": fpe["true_rsd"] / fpe["rhs_norm"],
                                 "rel_error_vs_fp64_solution": err(xe.download())}
-    for key, eo, gprec, label in (("mixed", False, w.F32, "F32 12-real links"),
-                                  ("mixed_even_odd", True, w.F16, "F16 12-real links")):
+    for key, eo, gprec, label in (("mixed", False, w.F32, "F32 12-real links" + TG_LABEL),
+                                  ("mixed_even_odd", True, w.F16, "F16 12-real links" + TG_LABEL)):
         if key not in variants:
             continue
-        Us = ctx.gauge(A, prec=gprec, recon=12)
+        Us = ctx.gauge(A, prec=gprec, recon=w.RECON_12T)
         for rep in range(2):  # the first call also autotunes the operator
             xm = ctx.field()
             mx = ctx.cg_mixed(xm, fb, U, Us, [mass], tol=1e-10, delta=DELTA, spinor_prec=w.F16,
@@ -261,7 +272,7 @@ def cg_compare(w, ctx, U, A, b, mass, max_iter=20000, variants=CG_VARIANTS):
                     "precision": f"FP32 arithmetic, F16 spinors, {label}, FP64 reliable updates"}
         del Us
     if "mixed_bicgstab" in variants:
-        Us = ctx.gauge(A, prec=w.F16, recon=12)
+        Us = ctx.gauge(A, prec=w.F16, recon=w.RECON_12T)
         for rep in range(2):  # the first call also autotunes the operator
             xm = ctx.field()
             mx = ctx.bicgstab_mixed(xm, fb, U, Us, [mass], tol=1e-10, delta=DELTA,
@@ -274,7 +285,8 @@ def cg_compare(w, ctx, U, A, b, mass, max_iter=20000, variants=CG_VARIANTS):
             "seconds": round(mx["seconds"], 4), "solves_per_s": round(1.0 / mx["seconds"], 3),
             "true_rel_residual": s["true_rel_rsd"],
             "rel_error_vs_fp64_solution": err(xm.download()),
-            "precision": "FP32 arithmetic, F16 spinors, F16 12-real links, FP64 reliable updates"}
+            "precision": "FP32 arithmetic, F16 spinors, F16 12-real links" + TG_LABEL
+                         + ", FP64 reliable updates"}
         out["speedup_mixed_bicgstab_vs_fp64"] = round(fp["seconds"] / mx["seconds"], 3)
         del Us
     mixed = [k for k in ("mixed", "mixed_even_odd") if k in out]
@@ -322,17 +334,20 @@ def loop_kernel_case(w, ctx, fb, U, A, fin, masses):
     BiCGStab solves on each (every system to 1e-10 on the full system)."""
     n = fin.nrhs
     V = w.volume(fin.ctx.L)
-    Us = ctx.gauge(A, prec=w.F16, recon=12)
+    Us = ctx.gauge(A, prec=w.F16, recon=w.RECON_12T)
     fout = ctx.field(prec=w.F16, nrhs=n)
     Rt = 1 if n == 1 else 2
-    out = {"links": "F16 12-real", "rhs_per_thread_compared": Rt}
+    out = {"links": "F16 12-real" + TG_LABEL, "rhs_per_thread_compared": Rt}
     for key, R in (("rhs2" if Rt == 2 else "rhs1", Rt), ("loop", w.RHS_LOOP)):
         full = ctx.time_sloppy(fout, fin, Us, masses, R, 0, -1, 10)
         half = ctx.time_sloppy(fout, fin, Us, masses, R, 0, 0, 10)
         out[f"M_ms_{key}"] = round(full, 4)
         out[f"one_parity_ms_{key}"] = round(half, 4)
-        # links once per site, input spinor twice (neighbours + diagonal), output once
-        out[f"M_GBps_{key}"] = round((8 * 24 / n + 3 * 52) * V * n / (full * 1e-3) / 1e9, 1)
+        # links once per site (t links on the boundary slices only), input
+        # spinor twice (neighbours + diagonal), output once
+        Lt = fin.ctx.L[3]
+        out[f"M_GBps_{key}"] = round(((6 + 2 / Lt) * 24 / n + 3 * 52) * V * n / (full * 1e-3)
+                                     / 1e9, 1)
         x = ctx.field(nrhs=n)
         cg = require_converged(ctx.cg_mixed(x, fb, U, Us, masses, tol=1e-10, rhs_per_thread=R,
                                             site_chunk=0, even_odd=True), f"Schur CG R={R}")
@@ -397,7 +412,7 @@ def batched32_case(w, L=(32, 32, 32, 64), n=12, kappa=0.135):
         for key, gprec, eo in (("mixed", w.F32, False), ("mixed_even_odd", w.F16, True),
                                ("mixed_bicgstab", w.F16, True)):
             log(f"batched {'x'.join(map(str, L))}: {key} of {n} systems")
-            Us = ctx.gauge(A, prec=gprec, recon=12)
+            Us = ctx.gauge(A, prec=gprec, recon=w.RECON_12T)
             if key == "mixed_bicgstab":
                 ctx.bicgstab_mixed(X, fb, U, Us, [mass] * n, tol=1e-10, delta=DELTA,
                                    spinor_prec=w.F16, max_iter=1)  # untimed: autotuning
@@ -418,7 +433,7 @@ def batched32_case(w, L=(32, 32, 32, 64), n=12, kappa=0.135):
                         "max_rel_error_vs_fp64_solution": max(errs),
                         "rhs_per_thread": mx["systems"][0]["rhs_per_thread"],
                         "site_chunk": mx["systems"][0]["site_chunk"],
-                        "links": "F32 12-real" if gprec == w.F32 else "F16 12-real"}
+                        "links": ("F32 12-real" if gprec == w.F32 else "F16 12-real") + TG_LABEL}
             del Us
     best = min(("mixed", "mixed_even_odd"), key=lambda k: out[k]["seconds"])
     out["speedup_mixed_vs_fp64"] = round(t_fp / out[best]["seconds"], 3)
@@ -445,7 +460,7 @@ def batched_case(w):
     n = len(srcs)
     with w.Context(L, bc=[1, 1, 1, -1]) as ctx:
         U = ctx.gauge(A)
-        Us = ctx.gauge(A, prec=w.F32, recon=12)
+        Us = ctx.gauge(A, prec=w.F32, recon=w.RECON_12T)
         fb = ctx.field(nrhs=n, data=srcs)
         x = ctx.field(nrhs=n)
         log("batched: mixed solve of 48 systems")
@@ -453,7 +468,7 @@ def batched_case(w):
         mx = require_converged(ctx.cg_mixed(x, fb, U, Us, masses, tol=1e-10, spinor_prec=w.F16),
                                "batched mixed CG")
         log("batched: mixed solve of 48 systems on the Schur system")
-        Us16 = ctx.gauge(A, prec=w.F16, recon=12)
+        Us16 = ctx.gauge(A, prec=w.F16, recon=w.RECON_12T)
         xe = ctx.field(nrhs=n)
         tune_solver(w, ctx, xe, fb, U, Us16, masses, even_odd=True)
         mxe = require_converged(ctx.cg_mixed(xe, fb, U, Us16, masses, tol=1e-10,
@@ -574,7 +589,8 @@ def link_product_case(w, ctx, nsys, products_per_M, ms_M):
 
 def large_case(w, full_system=False):
     """48^3x96, 12 Gaussian systems at kappa 0.135, F16 spinors and F16 12-real
-    links; the mixed CG on the full M^dag M (about 13 s) only with full_system."""
+    links in temporal gauge; the mixed CG on the full M^dag M (about 13 s) only
+    with full_system."""
     L = [48, 48, 48, 96]
     n = 12
     mass = kappa_mass(0.135)
@@ -582,7 +598,7 @@ def large_case(w, full_system=False):
     A = w.gauge_near_identity(L, 20211025, 0.3)
     with w.Context(L, bc=[1, 1, 1, -1]) as ctx:
         U = ctx.gauge(A)
-        Us = ctx.gauge(A, prec=w.F16, recon=12)
+        Us = ctx.gauge(A, prec=w.F16, recon=w.RECON_12T)
         fb = ctx.field(nrhs=n)
         for r in range(n):
             fb.upload(w.spinor_gaussian(L, 20211026, r), r)
@@ -628,10 +644,12 @@ def large_case(w, full_system=False):
         log("large: MFMA vs VALU link product")
         linkprod = link_product_case(w, ctx, n, 8 * w.volume(L) * n, ms_loop)
     V = w.volume(L)
-    bytes_site = 8 * 24 / n + 3 * 52   # F16 12-real links shared by n systems, F16 spinors
+    # F16 12-real links in temporal gauge shared by n systems, F16 spinors
+    bytes_site = (6 + 2 / L[3]) * 24 / n + 3 * 52
     cg_systems = mxe["systems"] + (mx["systems"] if mx else [])
     out = {
-        "workload": "48x48x48x96, 12 Gaussian systems, kappa 0.135, F16 spinors, F16 12-real links",
+        "workload": ("48x48x48x96, 12 Gaussian systems, kappa 0.135, F16 spinors, F16 12-real "
+                     "links" + TG_LABEL),
         "mixed_even_odd_seconds": round(mxe["seconds"], 3),
         "mixed_even_odd_solves_per_s": round(n / mxe["seconds"], 3),
         "fp64_seconds_per_solve": round(fp["seconds"], 3),
@@ -675,6 +693,7 @@ def profile_case(w, name, R=None):
     order (no autotuning), for rocprofv3 kernel statistics and counters:
       mixed32              32^3x64, one Gaussian system, mixed CG on M^dag M
                            (F16 spinors, F32 12-real links), delta 1e-2
+                           (every sloppy link field here in temporal gauge)
       eo32 / bicgstab32    the same system, mixed CG on the Schur system / mixed
                            BiCGStab (F16 spinors and F16 12-real links)
       batched32            32^3x64, 12 Gaussian systems, mixed CG on M^dag M
@@ -708,7 +727,8 @@ def profile_case(w, name, R=None):
     A = w.gauge_near_identity(L, 20211025, 0.3)
     with w.Context(L, bc=[1, 1, 1, -1]) as ctx:
         U = ctx.gauge(A)
-        Us = ctx.gauge(A, prec=w.F32 if name in ("mixed32", "batched32") else w.F16, recon=12)
+        Us = ctx.gauge(A, prec=w.F32 if name in ("mixed32", "batched32") else w.F16,
+                       recon=w.RECON_12T)
         if srcs is None:
             fb = ctx.field(nrhs=n)
             for r in range(n):
@@ -826,6 +846,10 @@ def main():
         best_R, best_C, ms_t = ctx.tune_report(s_in, Us)
         # one system: chunk has no effect, and the loop kernel is not a candidate
         ms_s = {R: ms_t[(R, 0)] for R in w.TUNE_R if (R, 0) in ms_t}
+        # the same links in temporal gauge (the mixed solvers' links here)
+        Ust = ctx.gauge(A, prec=w.F32, recon=w.RECON_12T)
+        ms_tg = ctx.time_sloppy(s_out, s_in, Ust, [mass], best_R, 0, -1, max(args.steps, 10))
+        del Ust
         if not args.no_cg:
             log("CG: FP64 vs mixed")
             extra["cg"] = cg_compare(w, ctx, U, A, b, mass, variants=args.cg)
@@ -885,6 +909,13 @@ def main():
                             "unit": "GB/s", "algorithmic_bytes_per_site": BYTES_M_SLOPPY,
                             "ms_per_launch": round(ms_s[best_R], 4),
                             "gflops": round(FLOP_M * V / (ms_s[best_R] * 1e-3) / 1e9, 1)},
+        "sloppy_temporal_gauge": {
+            "kernel": "k_hop_sloppy<F16 spinors, F32 12-real links in temporal gauge, HOP_M>",
+            "rhs_per_thread": best_R,
+            "achieved": round(bytes_m_sloppy_tg(L[3]) * V / (ms_tg * 1e-3) / 1e9, 1),
+            "frac": round(bytes_m_sloppy_tg(L[3]) * V / (ms_tg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "unit": "GB/s", "algorithmic_bytes_per_site": round(bytes_m_sloppy_tg(L[3]), 2),
+            "ms_per_launch": round(ms_tg, 4)},
     }
     if traffic_src:
         line["roofline"]["traffic_source"] = traffic_src

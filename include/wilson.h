@@ -48,6 +48,15 @@ extern "C" {
 
 #define WD_RECON_18 18 /* links stored as all 9 complex entries */
 #define WD_RECON_12 12 /* rows 0 and 1 stored; row 2 = conj(row0 x row1) (SU(3) only) */
+/* Sloppy links (F32, F16) stored 12-real in temporal gauge: U'_mu(x) =
+ * Omega(x) U_mu(x) Omega(x + mu)^dag with Omega(x, y, z, t) = U_t(t = 0) ...
+ * U_t(t - 1) along the site's t column (Omega = 1 on the first slice), so
+ * every t link but those of the last slice is the identity and the sloppy
+ * operators load and multiply t links only there (SU(3) links only).  The
+ * mixed solvers rotate their sloppy residuals by Omega and the corrections
+ * back by Omega^dag at the FP64 reliable updates; wd_apply_M_sloppy and
+ * wd_time_sloppy apply Omega M Omega^dag.  No counterpart in the reference. */
+#define WD_RECON_12T 13
 
 typedef struct wd_ctx wd_ctx;
 typedef struct wd_field wd_field;
@@ -78,7 +87,8 @@ int wd_release_workspace(wd_ctx *ctx);
 int wd_workspace_bytes(const wd_ctx *ctx, size_t *bytes);
 
 /* ---- gauge field (lattice_gauge, include/lattice_gauge.h:11-26) ---------- */
-/* prec: WD_F64 | WD_F32 | WD_F16 ; recon: WD_RECON_18 | WD_RECON_12.  The
+/* prec: WD_F64 | WD_F32 | WD_F16 ; recon: WD_RECON_18 | WD_RECON_12 (|
+ * WD_RECON_12T for F32 and F16: the mixed solvers' sloppy links).  The
  * FP64 operators and solvers (wd_hop, wd_apply_M, wd_apply_MdagM, wd_cg,
  * wd_cg_eo, the reliable updates of the mixed solvers) take F64 links in
  * either form: 18-real is the reference's format and applies any links;

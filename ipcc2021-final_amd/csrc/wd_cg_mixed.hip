@@ -52,6 +52,42 @@ using namespace wdf;
 
 namespace {
 
+// Temporal-gauge frame of the sloppy links (WD_RECON_12T): v <- Omega v
+// (dag = false: FP64 residual into the sloppy frame) or v <- Omega^dag v
+// (dag = true: sloppy correction back), Omega of site i of parity par, stored
+// complex [parity][9][stride].  omega == nullptr: the identity.
+template <typename T>
+__device__ __forceinline__ void omega_rotate(const double2 *__restrict__ omega, int par, int i,
+                                             int stride, bool dag, cpx<T> (&v)[4][3])
+{
+    if (!omega)
+        return;
+    const double2 *o = omega + (size_t)par * 9 * stride + i;
+    cpx<double> W[3][3];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const double2 e = o[(size_t)k * stride];
+        W[k / 3][k % 3] = {e.x, e.y};
+    }
+#pragma unroll
+    for (int sp = 0; sp < 4; ++sp) {
+        cpx<double> r[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            cpx<double> acc = {0.0, 0.0};
+#pragma unroll
+            for (int b = 0; b < 3; ++b) {
+                const cpx<double> vb = {(double)v[sp][b].r, (double)v[sp][b].i};
+                acc = dag ? cfma_conj(acc, W[b][a], vb) : cfma(acc, W[a][b], vb);
+            }
+            r[a] = acc;
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            v[sp][a] = {(T)r[a].r, (T)r[a].i};
+    }
+}
+
 // The elementwise kernels stream whole fields with no reuse inside a launch:
 // non-temporal loads and stores keep them from churning L2.
 constexpr bool kStreamNT = true;
@@ -161,11 +197,12 @@ __global__ __launch_bounds__(kBlock) void k_mx_update(
     block_reduce_store(acc, 0.0, partials + (size_t)rhs * gridDim.x + blockIdx.x);
 }
 
-// x (F64) += x_s (F32); x_s = 0
+// x (F64) += x_s (F32); x_s = 0.  omega: the sloppy frame (x += Omega^dag x_s)
 __global__ __launch_bounds__(kBlock) void k_mx_relx(double2 *__restrict__ x,
                                                     char *__restrict__ xs, size_t x_rhs_elems,
                                                     size_t xs_rhs_bytes, int stride, int nel,
-                                                    const int *__restrict__ reliable)
+                                                    const int *__restrict__ reliable,
+                                                    const double2 *__restrict__ omega)
 {
     const int rhs = blockIdx.y;
     if (!reliable[rhs])
@@ -180,6 +217,7 @@ __global__ __launch_bounds__(kBlock) void k_mx_relx(double2 *__restrict__ x,
     cpx<float> vs[4][3], z[4][3];
     Spinor<WD_F64>::load(xp, nullptr, stride, (unsigned)i, vx);
     Spinor<WD_F32>::load(sp, nullptr, stride, (unsigned)i, vs);
+    omega_rotate(omega, par, i, stride, true, vs);
 #pragma unroll
     for (int s = 0; s < 4; ++s)
 #pragma unroll
@@ -192,7 +230,8 @@ __global__ __launch_bounds__(kBlock) void k_mx_relx(double2 *__restrict__ x,
     Spinor<WD_F32>::store(sp, nullptr, stride, (unsigned)i, z);
 }
 
-// r = bp - q64 (F64) -> r_s (SF); partial |r|^2.  With reliable == nullptr every rhs runs
+// r = bp - q64 (F64) -> r_s (SF, rotated into the sloppy frame by omega when
+// not null); partial |r|^2.  With reliable == nullptr every rhs runs
 // (initialisation).
 template <int SF>
 __global__ __launch_bounds__(kBlock) void k_mx_relr(char *__restrict__ rsl, float *__restrict__ rss,
@@ -201,7 +240,8 @@ __global__ __launch_bounds__(kBlock) void k_mx_relr(char *__restrict__ rsl, floa
                                                     size_t x_rhs_elems, size_t rhs_bytes,
                                                     int stride, int nel,
                                                     const int *__restrict__ reliable,
-                                                    double2 *__restrict__ partials)
+                                                    double2 *__restrict__ partials,
+                                                    const double2 *__restrict__ omega)
 {
     const int rhs = blockIdx.y;
     if (reliable && !reliable[rhs])
@@ -230,6 +270,7 @@ __global__ __launch_bounds__(kBlock) void k_mx_relr(char *__restrict__ rsl, floa
             }
         const size_t off = rhs * rhs_bytes + (size_t)par * spinor_chunks(SF) * 16 * stride;
         const size_t so = ((size_t)rhs * 2 + par) * stride;
+        omega_rotate(omega, par, i, stride, false, vb);
         Spinor<SF>::store(rsl + off, rss ? rss + so : nullptr, stride, (unsigned)i, vb);
     }
     block_reduce_store(acc, 0.0, partials + (size_t)rhs * gridDim.x + blockIdx.x);
@@ -612,7 +653,7 @@ __global__ __launch_bounds__(kBlock) void k_bs_relr(
     char *__restrict__ r, float *__restrict__ rs, char *__restrict__ r0, float *__restrict__ r0s,
     bool r0w, const double2 *__restrict__ bp, const double2 *__restrict__ q64, size_t x_rhs_elems,
     size_t rhs_bytes, int stride, int nel, const int *__restrict__ mask, double2 *__restrict__ pa,
-    double2 *__restrict__ pb)
+    double2 *__restrict__ pb, const double2 *__restrict__ omega)
 {
     const int rhs = blockIdx.y;
     if (mask && !mask[rhs])
@@ -639,6 +680,7 @@ __global__ __launch_bounds__(kBlock) void k_bs_relr(
                 rr = fma(vb[s][c].r, vb[s][c].r, rr);
                 rr = fma(vb[s][c].i, vb[s][c].i, rr);
             }
+        omega_rotate(omega, par, i, stride, false, vb);
         Spinor<SF>::store(r + off, rs ? rs + so : nullptr, stride, (unsigned)i, vb);
         if (r0w)
             Spinor<SF>::store(r0 + off, r0s ? r0s + so : nullptr, stride, (unsigned)i, vb);
@@ -1147,11 +1189,13 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
     if (sp == WD_F16)
         hipLaunchKernelGGL(k_mx_relr<WD_F16>, dim3(eg, n), dim3(kBlock), 0, ctx->stream,
                            (char *)rs.f->data, rs.f->scale, (const double2 *)Mdbd,
-                           (const double2 *)nullptr, xe, srb, stride, nel, (const int *)nullptr, part);
+                           (const double2 *)nullptr, xe, srb, stride, nel, (const int *)nullptr, part,
+                           (const double2 *)Us->omega);
     else
         hipLaunchKernelGGL(k_mx_relr<WD_F32>, dim3(eg, n), dim3(kBlock), 0, ctx->stream,
                            (char *)rs.f->data, rs.f->scale, (const double2 *)Mdbd,
-                           (const double2 *)nullptr, xe, srb, stride, nel, (const int *)nullptr, part);
+                           (const double2 *)nullptr, xe, srb, stride, nel, (const int *)nullptr, part,
+                           (const double2 *)Us->omega);
     if ((rc = wd_check_launch(ctx, "k_mx_relr")))
         return rc;
     // even-odd: the Schur target starts at half the full-system tolerance
@@ -1267,7 +1311,8 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
     auto reliable_slot = [&]() -> int {
         int e;
         hipLaunchKernelGGL(k_mx_relx, dim3(eg, n), dim3(kBlock), 0, ctx->stream, xd,
-                           (char *)xs.f->data, xe, xsb, stride, nel, (const int *)reliable);
+                           (char *)xs.f->data, xe, xsb, stride, nel, (const int *)reliable,
+                           (const double2 *)Us->omega);
         if ((e = wd_check_launch(ctx, "k_mx_relx")))
             return e;
         if (!eo) {
@@ -1285,12 +1330,12 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
             hipLaunchKernelGGL(k_mx_relr<WD_F16>, dim3(eg, n), dim3(kBlock), 0, ctx->stream,
                                (char *)rs.f->data, rs.f->scale, (const double2 *)Mdbd,
                                (const double2 *)q64d, xe, srb, stride, nel, (const int *)reliable,
-                               part);
+                               part, (const double2 *)Us->omega);
         else
             hipLaunchKernelGGL(k_mx_relr<WD_F32>, dim3(eg, n), dim3(kBlock), 0, ctx->stream,
                                (char *)rs.f->data, rs.f->scale, (const double2 *)Mdbd,
                                (const double2 *)q64d, xe, srb, stride, nel, (const int *)reliable,
-                               part);
+                               part, (const double2 *)Us->omega);
         hipLaunchKernelGGL(k_mx_relfin, dim3(n), dim3(1024), 0, ctx->stream,
                            (const double2 *)part, eg, st, active, reliable, prm->max_iter);
         hipLaunchKernelGGL(k_mx_alldone, dim3(1), dim3(64), 0, ctx->stream,
@@ -1560,12 +1605,12 @@ extern "C" int wd_bicgstab_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, co
         hipLaunchKernelGGL(k_bs_relr<WD_F16>, dim3(eg, n), dim3(kBlock), 0, ctx->stream,
                            (char *)r.f->data, r.f->scale, (char *)r0.f->data, r0.f->scale, true,
                            (const double2 *)bhd, (const double2 *)nullptr, xe, srb, stride, nel,
-                           (const int *)nullptr, pa, pb);
+                           (const int *)nullptr, pa, pb, (const double2 *)Us->omega);
     else
         hipLaunchKernelGGL(k_bs_relr<WD_F32>, dim3(eg, n), dim3(kBlock), 0, ctx->stream,
                            (char *)r.f->data, r.f->scale, (char *)r0.f->data, r0.f->scale, true,
                            (const double2 *)bhd, (const double2 *)nullptr, xe, srb, stride, nel,
-                           (const int *)nullptr, pa, pb);
+                           (const int *)nullptr, pa, pb, (const double2 *)Us->omega);
     // the Schur target starts at half the full-system tolerance (relative to |bh|)
     hipLaunchKernelGGL(k_bs_init, dim3(n), dim3(1024), 0, ctx->stream, (const double2 *)pa,
                        (const double2 *)pb, eg, st, active, reliable, 0.5 * prm->tol);
@@ -1678,13 +1723,14 @@ extern "C" int wd_bicgstab_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, co
     auto reliable_slot = [&]() -> int {
         int e;
         hipLaunchKernelGGL(k_mx_relx, dim3(eg, n), dim3(kBlock), 0, ctx->stream, xd,
-                           (char *)xs.f->data, xe, xsb, stride, nel, (const int *)reliable);
+                           (char *)xs.f->data, xe, xsb, stride, nel, (const int *)reliable,
+                           (const double2 *)Us->omega);
         if ((e = wd_check_launch(ctx, "k_mx_relx")) ||
             (e = apply_A64(xd, t64d, rel, rel2, stop)))
             return e;
         BS_SF(k_bs_relr, (char *)r.f->data, r.f->scale, (char *)r0.f->data, r0.f->scale, false,
               (const double2 *)bhd, (const double2 *)t64d, xe, srb, stride, nel,
-              (const int *)reliable, pa, pb);
+              (const int *)reliable, pa, pb, (const double2 *)Us->omega);
         hipLaunchKernelGGL(k_bs_relfin, dim3(n), dim3(1024), 0, ctx->stream, (const double2 *)pa,
                            (const double2 *)pb, eg, st, active, reliable, prm->max_iter);
         hipLaunchKernelGGL(k_mx_alldone, dim3(1), dim3(64), 0, ctx->stream, (const int *)active,

@@ -69,19 +69,11 @@ __global__ __launch_bounds__(kBlock) void k_spinor_convert(char *__restrict__ ds
                       dscale ? dscale + (size_t)p * stride : nullptr, stride, (unsigned)i, v);
 }
 
-// reference links A[mu] complex<double>[V*9] -> device gauge format GF
+// One link (9 complex, row-major) stored at site i of a (mu, parity) block of
+// device gauge format GF (12-real formats keep rows 0 and 1).
 template <int GF>
-__global__ __launch_bounds__(kBlock) void k_gauge_from_ref(char *__restrict__ dst,
-                                                           const double2 *__restrict__ src,
-                                                           int mu, int Vh, int stride)
+__device__ __forceinline__ void store_link(char *base, int stride, int i, const double2 (&s)[9])
 {
-    const int e = blockIdx.x * kBlock + threadIdx.x;
-    if (e >= 2 * Vh)
-        return;
-    const int p = e >= Vh, i = e - p * Vh;
-    const size_t block = (size_t)gauge_chunks(GF) * gauge_chunk_bytes(GF) * stride;
-    char *base = dst + (size_t)(mu * 2 + p) * block;
-    const double2 *s = src + (size_t)e * 9;
     if constexpr (GF == G_F64_18 || GF == G_F64_12) {
         double2 *b = reinterpret_cast<double2 *>(base);
         for (int k = 0; k < gauge_chunks(GF); ++k)
@@ -102,6 +94,121 @@ __global__ __launch_bounds__(kBlock) void k_gauge_from_ref(char *__restrict__ ds
                 make_uint2(f2h((float)s[2 * q].x, (float)s[2 * q].y),
                            f2h((float)s[2 * q + 1].x, (float)s[2 * q + 1].y));
     }
+}
+
+// reference links A[mu] complex<double>[V*9] -> device gauge format GF
+template <int GF>
+__global__ __launch_bounds__(kBlock) void k_gauge_from_ref(char *__restrict__ dst,
+                                                           const double2 *__restrict__ src,
+                                                           int mu, int Vh, int stride)
+{
+    const int e = blockIdx.x * kBlock + threadIdx.x;
+    if (e >= 2 * Vh)
+        return;
+    const int p = e >= Vh, i = e - p * Vh;
+    const size_t block = (size_t)gauge_chunks(GF) * gauge_chunk_bytes(GF) * stride;
+    double2 s[9];
+    for (int k = 0; k < 9; ++k)
+        s[k] = src[(size_t)e * 9 + k];
+    store_link<GF>(dst + (size_t)(mu * 2 + p) * block, stride, i, s);
+}
+
+// ---- temporal gauge (WD_RECON_12T) --------------------------------------------
+// Omega(x, y, z, t) = U_t(t = 0) U_t(t = 1) ... U_t(t - 1) along each t column
+// (Omega = 1 on slice 0), so that Omega(x) U_t(x) Omega(x + t)^dag = 1 off the
+// last slice.  One thread per column walks t; FP64 products.  omega: complex
+// [parity][9][stride]; ut: the t links in ref layout ([parity][site][9]).
+__global__ __launch_bounds__(kBlock) void k_tg_omega(double2 *__restrict__ omega,
+                                                     const double2 *__restrict__ ut, Geom g)
+{
+    const int Lx = 2 * g.Lxh;
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= Lx * g.Ly * g.Lz)
+        return;
+    const int x = c % Lx, y = (c / Lx) % g.Ly, z = c / (Lx * g.Ly);
+    cpx<double> W[3][3];
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+            W[a][b] = {a == b ? 1.0 : 0.0, 0.0};
+    for (int t = 0; t < g.Lt; ++t) {
+        const int p = (x + y + z + t) & 1;
+        const int i = (x >> 1) + g.Lxh * (y + g.Ly * (z + g.Lz * t));
+        double2 *o = omega + (size_t)p * 9 * g.stride;
+        for (int k = 0; k < 9; ++k)
+            o[(size_t)k * g.stride + i] = make_double2(W[k / 3][k % 3].r, W[k / 3][k % 3].i);
+        if (t + 1 == g.Lt)
+            break;
+        const double2 *u = ut + ((size_t)p * g.Vh + i) * 9;
+        cpx<double> N[3][3];
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) {
+                cpx<double> acc = {0.0, 0.0};
+                for (int k = 0; k < 3; ++k)
+                    acc = cfma(acc, W[a][k], cpx<double>{u[3 * k + b].x, u[3 * k + b].y});
+                N[a][b] = acc;
+            }
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                W[a][b] = N[a][b];
+    }
+}
+
+// U'_mu(x) = Omega(x) U_mu(x) Omega(x + mu)^dag in FP64, stored in format GF
+template <int GF>
+__global__ __launch_bounds__(kBlock) void k_tg_links(char *__restrict__ dst,
+                                                     const double2 *__restrict__ src,
+                                                     const double2 *__restrict__ omega, int mu,
+                                                     Geom g)
+{
+    const int e = blockIdx.x * kBlock + threadIdx.x;
+    if (e >= 2 * g.Vh)
+        return;
+    const int p = e >= g.Vh, i = e - p * g.Vh;
+    int rr = i / g.Lxh;
+    const int xh = i - rr * g.Lxh;
+    const int y = rr % g.Ly;
+    rr /= g.Ly;
+    const int z = rr % g.Lz;
+    const int t = rr / g.Lz;
+    const int x = 2 * xh + ((p + y + z + t) & 1);
+    int j; // x + mu, of the other parity
+    if (mu == 0) {
+        const int xf = (x + 1 == 2 * g.Lxh) ? 0 : x + 1;
+        j = i - xh + (xf >> 1);
+    } else if (mu == 1) {
+        j = (y + 1 == g.Ly) ? i - (g.Ly - 1) * g.Lxh : i + g.Lxh;
+    } else if (mu == 2) {
+        const int s = g.Lxh * g.Ly;
+        j = (z + 1 == g.Lz) ? i - (g.Lz - 1) * s : i + s;
+    } else {
+        const int s = g.Lxh * g.Ly * g.Lz;
+        j = (t + 1 == g.Lt) ? i - (g.Lt - 1) * s : i + s;
+    }
+    const double2 *oa = omega + (size_t)p * 9 * g.stride;
+    const double2 *ob = omega + (size_t)(1 - p) * 9 * g.stride;
+    cpx<double> A[3][3], B[3][3], Uu[3][3];
+    for (int k = 0; k < 9; ++k) {
+        const double2 va = oa[(size_t)k * g.stride + i], vb = ob[(size_t)k * g.stride + j];
+        const double2 vu = src[(size_t)e * 9 + k];
+        A[k / 3][k % 3] = {va.x, va.y};
+        B[k / 3][k % 3] = {vb.x, vb.y};
+        Uu[k / 3][k % 3] = {vu.x, vu.y};
+    }
+    double2 s[9];
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+            // (A U B^dag)_ab = sum_c (A U)_ac conj(B_bc)
+            cpx<double> acc = {0.0, 0.0};
+            for (int c = 0; c < 3; ++c) {
+                cpx<double> au = {0.0, 0.0};
+                for (int k = 0; k < 3; ++k)
+                    au = cfma(au, A[a][k], Uu[k][c]);
+                acc = cfma_conj(acc, B[b][c], au);
+            }
+            s[3 * a + b] = make_double2(acc.r, acc.i);
+        }
+    const size_t block = (size_t)gauge_chunks(GF) * gauge_chunk_bytes(GF) * g.stride;
+    store_link<GF>(dst + (size_t)(mu * 2 + p) * block, g.stride, i, s);
 }
 
 template <int FMT>
@@ -195,6 +302,28 @@ int wd_launch_gauge_from_ref(wd_ctx *ctx, wd_gauge *U, int mu, const double2 *sr
         break;
     }
     return wd_check_launch(ctx, "k_gauge_from_ref");
+}
+
+int wd_launch_tg_omega(wd_ctx *ctx, wd_gauge *U, const double2 *src)
+{
+    const Geom &g = ctx->geom;
+    const int cols = 2 * g.Lxh * g.Ly * g.Lz;
+    hipLaunchKernelGGL(k_tg_omega, dim3((cols + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream,
+                       U->omega, src, g);
+    return wd_check_launch(ctx, "k_tg_omega");
+}
+
+int wd_launch_tg_links(wd_ctx *ctx, wd_gauge *U, int mu, const double2 *src)
+{
+    const int grid = (int)((2 * ctx->Vh + kBlock - 1) / kBlock);
+    char *d = (char *)U->data;
+    if (gauge_fmt(U->prec, U->recon) == G_F32_12)
+        hipLaunchKernelGGL(k_tg_links<G_F32_12>, dim3(grid), dim3(kBlock), 0, ctx->stream, d, src,
+                           (const double2 *)U->omega, mu, ctx->geom);
+    else
+        hipLaunchKernelGGL(k_tg_links<G_F16_12>, dim3(grid), dim3(kBlock), 0, ctx->stream, d, src,
+                           (const double2 *)U->omega, mu, ctx->geom);
+    return wd_check_launch(ctx, "k_tg_links");
 }
 
 // dst[rd] = src[rs], converting between any two spinor formats

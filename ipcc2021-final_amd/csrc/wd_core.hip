@@ -249,9 +249,12 @@ extern "C" int wd_gauge_create(wd_ctx *ctx, wd_gauge **U, int prec, int recon)
         return WD_EARG;
     *U = nullptr;
     WD_ARG_CHECK(ctx, prec == WD_F64 || prec == WD_F32 || prec == WD_F16, "gauge: bad precision");
-    WD_ARG_CHECK(ctx, recon == WD_RECON_18 || recon == WD_RECON_12, "gauge: bad recon");
+    WD_ARG_CHECK(ctx, recon == WD_RECON_18 || recon == WD_RECON_12 || recon == WD_RECON_12T,
+                 "gauge: bad recon");
     WD_ARG_CHECK(ctx, !(prec == WD_F16 && recon == WD_RECON_18),
                  "gauge: F16 links are stored 12-real only");
+    WD_ARG_CHECK(ctx, !(prec == WD_F64 && recon == WD_RECON_12T),
+                 "gauge: temporal-gauge links are for the sloppy formats (F32, F16)");
     wd_gauge *g = new (std::nothrow) wd_gauge();
     if (!g)
         return WD_ENOMEM;
@@ -264,7 +267,14 @@ extern "C" int wd_gauge_create(wd_ctx *ctx, wd_gauge **U, int prec, int recon)
         delete g;
         return wd_set_error(ctx, WD_ENOMEM, "gauge allocation failed");
     }
+    const size_t obytes = sizeof(double2) * 2 * 9 * ctx->stride;
+    if (recon == WD_RECON_12T && wd_malloc(ctx, (void **)&g->omega, obytes) != hipSuccess) {
+        wd_gauge_destroy(g);
+        return wd_set_error(ctx, WD_ENOMEM, "gauge allocation failed");
+    }
     int rc = wd_launch_zero(ctx, g->data, g->bytes);
+    if (!rc && g->omega)
+        rc = wd_launch_zero(ctx, g->omega, obytes);
     if (rc) {
         wd_gauge_destroy(g);
         return rc;
@@ -279,6 +289,8 @@ extern "C" void wd_gauge_destroy(wd_gauge *U)
         return;
     (void)hipStreamSynchronize(U->ctx->stream);
     (void)hipFree(U->data);
+    if (U->omega)
+        (void)hipFree(U->omega);
     delete U;
 }
 
@@ -290,11 +302,21 @@ extern "C" int wd_gauge_upload_ref(wd_ctx *ctx, wd_gauge *U, const double *const
     int rc = wd_ensure_staging(ctx, bytes);
     if (rc)
         return rc;
-    for (int mu = 0; mu < 4; ++mu) {
+    for (int mu = 0; mu < 4; ++mu)
         WD_ARG_CHECK(ctx, A[mu] != nullptr, "gauge upload: null direction array");
+    const bool tg = U->recon == WD_RECON_12T;
+    // temporal gauge: the t links first (Omega), then every direction rotated
+    for (int k = 0; k < 4; ++k) {
+        const int mu = tg ? (k + 3) % 4 : k;
         WD_HIP_CHECK(ctx, hipMemcpyAsync(ctx->staging, A[mu], bytes, hipMemcpyHostToDevice,
                                          ctx->stream));
-        rc = wd_launch_gauge_from_ref(ctx, U, mu, (const double2 *)ctx->staging);
+        const double2 *src = (const double2 *)ctx->staging;
+        if (tg) {
+            rc = mu == 3 ? wd_launch_tg_omega(ctx, U, src) : 0;
+            if (!rc)
+                rc = wd_launch_tg_links(ctx, U, mu, src);
+        } else
+            rc = wd_launch_gauge_from_ref(ctx, U, mu, src);
         if (rc)
             return rc;
     }

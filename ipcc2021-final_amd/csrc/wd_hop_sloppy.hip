@@ -55,12 +55,14 @@ __device__ __forceinline__ void pin_accs(cpx<float> (&acc)[R][4][3])
 
 // One hop direction for the R right-hand sides of this thread.  in_b / isc_b:
 // per-rhs base of the neighbour parity (spinor data, F16 scales); hs = -1/2
-// times the boundary phase of this hop.
+// times the boundary phase of this hop; unit: the link is the identity (t
+// links of a temporal-gauge field off the last slice), neither loaded nor
+// applied.
 template <int SF, int GF, int R, int MU, bool FWD, bool DAG>
 __device__ __forceinline__ void dir_rhs(cpx<float> (&acc)[R][4][3], const SloppyArgs &a,
                                         const bool (&act)[R], const char *const (&in_b)[R],
                                         const float *const (&isc_b)[R], int p, int i, int j,
-                                        float hs, int stride)
+                                        float hs, int stride, bool unit = false)
 {
     cpx<float> W[3][3];
     // forward: U_mu(x) at (parity p, site i); backward: U_mu(x - mu) at (1 - p, j)
@@ -69,10 +71,17 @@ __device__ __forceinline__ void dir_rhs(cpx<float> (&acc)[R][4][3], const Sloppy
     const char *Ub = a.U + (size_t)(MU * 2 + lpar) * blk;
     // links read once per launch stream past L2; temporal when the next
     // groups of this site range, or the other parity, re-read them (uniform)
-    if (a.tlinks)
+    if (unit) {
+#pragma unroll
+        for (int x = 0; x < 3; ++x)
+#pragma unroll
+            for (int y = 0; y < 3; ++y)
+                W[x][y] = {0.f, 0.f};
+    } else if (a.tlinks) {
         load_link32<GF, false>(Ub, stride, (unsigned)(FWD ? i : j), W);
-    else
+    } else {
         load_link32<GF, true>(Ub, stride, (unsigned)(FWD ? i : j), W);
+    }
     using D = Dir<MU, FWD, DAG>;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -94,8 +103,12 @@ __device__ __forceinline__ void dir_rhs(cpx<float> (&acc)[R][4][3], const Sloppy
             load32(in_b[r], stride, (unsigned)j, psi);
             project<D>(psi, h);
         }
-        su3_mul_half<!FWD>(W, h, res);
-        reconstruct_f<D>(acc[r], res, f);
+        if (unit) {
+            reconstruct_f<D>(acc[r], h, f);
+        } else {
+            su3_mul_half<!FWD>(W, h, res);
+            reconstruct_f<D>(acc[r], res, f);
+        }
     }
     pin_accs<R>(acc);
 }
@@ -207,10 +220,13 @@ __global__ __launch_bounds__(kBlock) void k_hop_sloppy(SloppyArgs a, Geom g, int
             const int s = g.Lxh * g.Ly * g.Lz;
             const int jf = (t + 1 == g.Lt) ? i - (g.Lt - 1) * s : i + s;
             const int jb = (t == 0) ? i + (g.Lt - 1) * s : i - s;
+            // temporal gauge: the forward t link is the identity off the last
+            // slice, the backward one (U_t of slice t - 1) off the first
             dir_rhs<SF, GF, R, 3, true, DAG>(acc, a, act, in_b, isc_b, p, i, jf,
-                                             t == g.Lt - 1 ? ht : h0, stride);
+                                             t == g.Lt - 1 ? ht : h0, stride,
+                                             a.tg && t != g.Lt - 1);
             dir_rhs<SF, GF, R, 3, false, DAG>(acc, a, act, in_b, isc_b, p, i, jb,
-                                              t == 0 ? ht : h0, stride);
+                                              t == 0 ? ht : h0, stride, a.tg && t != 0);
         }
     }
 
@@ -388,14 +404,15 @@ void launch_sf(wd_ctx *ctx, const SloppyArgs &a, const Geom &g, int gf, int R, i
 bool wd_sloppy_supported(int spinor_prec, int gauge_prec, int recon)
 {
     const bool s = spinor_prec == WD_F32 || spinor_prec == WD_F16;
-    const bool g = (gauge_prec == WD_F32) || (gauge_prec == WD_F16 && recon == WD_RECON_12);
+    const bool r12 = recon == WD_RECON_12 || recon == WD_RECON_12T;
+    const bool g = (gauge_prec == WD_F32) || (gauge_prec == WD_F16 && r12);
     return s && g;
 }
 
 bool wd_sloppy_loop_supported(int spinor_prec, int gauge_prec, int recon)
 {
     return spinor_prec == WD_F16 && (gauge_prec == WD_F16 || gauge_prec == WD_F32) &&
-           recon == WD_RECON_12;
+           (recon == WD_RECON_12 || recon == WD_RECON_12T);
 }
 
 int wd_sloppy_groups(int nrhs, int R) { return R == WD_RHS_LOOP ? 1 : (nrhs + R - 1) / R; }
@@ -424,6 +441,7 @@ int wd_launch_hop_sloppy(wd_ctx *ctx, SloppyArgs a, int spinor_prec, int gauge_p
     WD_ARG_CHECK(ctx, mode != HOP_M_NORM || (!dagger && a.partials),
                  "sloppy hop: the norm-only mode has no dagger form and needs partials");
     a.one = 1.0f;
+    a.tg = recon == WD_RECON_12T;
     WD_ARG_CHECK(ctx, wd_sloppy_supported(spinor_prec, gauge_prec, recon),
                  "sloppy hop: unsupported precision combination");
     const int nparity = (cb < 0) ? 2 : 1;

@@ -197,6 +197,24 @@ __device__ __forceinline__ void dir_math(cpx<float> (&acc)[4][3], const unsigned
     pin_acc(acc);
 }
 
+// The identity link in the register forms of the two link formats
+__device__ __forceinline__ void unit_link(unsigned (&W)[3][3])
+{
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+            W[a][b] = a == b ? 0x3c00u : 0u; // (1.0, 0.0) as packed halves
+}
+__device__ __forceinline__ void unit_link(cpx<float> (&W)[3][3])
+{
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+            W[a][b] = {a == b ? 1.f : 0.f, 0.f};
+}
+
 // Link formats of the loop kernel: F16 12-real links kept as packed halves
 // (72 registers, products by packed dot products), or F32 12-real links kept
 // as FP32 complex with row 2 rebuilt (144 registers, the FP32 products of the
@@ -358,30 +376,42 @@ __device__ __forceinline__ void loop_block(const SloppyArgs &a, const Geom &g, i
         for (int d = 0; d < 8; ++d)
             if (cross[d] && g.bc[d >> 1] < 0.0)
                 sbits |= 1u << d;
+        // temporal gauge: the forward t link is the identity off the last
+        // slice, the backward one (U_t of slice t - 1) off the first; such
+        // links are not loaded but set to 1 in registers (links load once
+        // per pass here, so a separate unit-link path would only cost
+        // registers: measured, it made the hop launches spill)
+        unsigned tbits = 0; // bit 0 forward, bit 1 backward t link is 1
+        if (a.tg)
+            tbits = (t != g.Lt - 1 ? 1u : 0u) | (t != 0 ? 2u : 0u);
         // 12-real links, 3 chunks per site: 8 bytes each (F16), 16 (F32)
         const size_t lblk = (size_t)3 * (LF == LL_F16 ? 8 : 16) * stride;
+        auto load_dir = [&](int d, const char *base, unsigned idx) {
+            if constexpr (LF == LL_F16) {
+                if (a.tlinks)
+                    load_link_h2<false>(base, stride, idx, L.W[d]);
+                else
+                    load_link_h2<true>(base, stride, idx, L.W[d]);
+            } else {
+                if (a.tlinks)
+                    load_link32<G_F32_12, false>(base, stride, idx, L.W[d]);
+                else
+                    load_link32<G_F32_12, true>(base, stride, idx, L.W[d]);
+            }
+        };
 #pragma unroll
         for (int mu = 0; mu < 4; ++mu) {
             // forward: U_mu(x) at (p, i); backward: U_mu(x - mu) at (1 - p, j)
             const char *Uf = a.U + (size_t)(mu * 2 + p) * lblk;
             const char *Ub = a.U + (size_t)(mu * 2 + 1 - p) * lblk;
-            if constexpr (LF == LL_F16) {
-                if (a.tlinks) {
-                    load_link_h2<false>(Uf, stride, (unsigned)i, L.W[2 * mu]);
-                    load_link_h2<false>(Ub, stride, jn[2 * mu + 1], L.W[2 * mu + 1]);
-                } else {
-                    load_link_h2<true>(Uf, stride, (unsigned)i, L.W[2 * mu]);
-                    load_link_h2<true>(Ub, stride, jn[2 * mu + 1], L.W[2 * mu + 1]);
-                }
-            } else {
-                if (a.tlinks) {
-                    load_link32<G_F32_12, false>(Uf, stride, (unsigned)i, L.W[2 * mu]);
-                    load_link32<G_F32_12, false>(Ub, stride, jn[2 * mu + 1], L.W[2 * mu + 1]);
-                } else {
-                    load_link32<G_F32_12, true>(Uf, stride, (unsigned)i, L.W[2 * mu]);
-                    load_link32<G_F32_12, true>(Ub, stride, jn[2 * mu + 1], L.W[2 * mu + 1]);
-                }
-            }
+            if (mu < 3 || !(tbits & 1u))
+                load_dir(2 * mu, Uf, (unsigned)i);
+            else
+                unit_link(L.W[2 * mu]);
+            if (mu < 3 || !(tbits & 2u))
+                load_dir(2 * mu + 1, Ub, jn[2 * mu + 1]);
+            else
+                unit_link(L.W[2 * mu + 1]);
         }
     }
 

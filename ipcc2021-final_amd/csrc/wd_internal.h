@@ -85,6 +85,9 @@ struct wd_gauge {
     int recon;
     void *data;
     size_t bytes;
+    // WD_RECON_12T: the rotation Omega of the temporal gauge, FP64 complex
+    // [parity][9][stride] (row-major 3x3 per site); nullptr otherwise
+    double2 *omega;
 };
 
 // ---- device memory ----------------------------------------------------------
@@ -197,6 +200,10 @@ float *wd_rhs_scale(const wd_field *f, int rhs);
 int wd_launch_spinor_from_ref(wd_ctx *ctx, wd_field *f, int rhs, const double2 *src);
 int wd_launch_spinor_to_ref(wd_ctx *ctx, const wd_field *f, int rhs, double2 *dst);
 int wd_launch_gauge_from_ref(wd_ctx *ctx, wd_gauge *U, int mu, const double2 *src);
+// Temporal gauge (WD_RECON_12T): Omega from the t links in ref layout
+// (src), then the rotated links of direction mu from src into U.
+int wd_launch_tg_omega(wd_ctx *ctx, wd_gauge *U, const double2 *src);
+int wd_launch_tg_links(wd_ctx *ctx, wd_gauge *U, int mu, const double2 *src);
 int wd_convert_rhs(wd_ctx *ctx, wd_field *dst, int rd, const wd_field *src, int rs);
 
 // batch description for the FP64 kernel: rhs r = blockIdx.y at element offset

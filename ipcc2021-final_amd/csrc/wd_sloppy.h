@@ -29,6 +29,8 @@ struct SloppyArgs {
     // filled by the launcher
     int groups, bpp, nsb;
     int tlinks;              // temporal link loads (links re-read within the launch)
+    int tg;                  // links in temporal gauge (WD_RECON_12T): t links are the
+                             // identity except on the last slice
 };
 
 bool wd_sloppy_supported(int spinor_prec, int gauge_prec, int recon);

@@ -38,6 +38,8 @@ OPT_LOOP_PREFETCH = 6
 OPT_LOOP_SYSTEMS = 7
 LINKPROD_VALU, LINKPROD_MFMA = 0, 1
 RECON_18, RECON_12 = 18, 12
+# sloppy (F32, F16) links stored 12-real in temporal gauge (include/wilson.h WD_RECON_12T)
+RECON_12T = 13
 # autotuner candidates (include/wilson.h WD_TUNE_NR x WD_TUNE_NC)
 TUNE_R = (1, 2, 4, RHS_LOOP)
 TUNE_CHUNK = (0, 16, 32, 64)
