@@ -111,11 +111,15 @@ __device__ __forceinline__ double2 sum_rhs_partials(const double2 *partials, int
 }
 
 // ---- elementwise kernels, blockIdx.y = rhs, one thread per (parity, site) ----
+// x_s += alpha p (the previous iteration's step, deferred from its update:
+// this pass reads p anyway), then p = r + beta p.  alpha is 0 when no step is
+// pending (first iteration, after a reliable update, which applies it).
 template <int SF>
 __global__ __launch_bounds__(kBlock) void k_mx_xpay(char *__restrict__ p, float *__restrict__ ps,
                                                     const char *__restrict__ r,
                                                     const float *__restrict__ rs,
-                                                    size_t rhs_bytes, int stride, int nel,
+                                                    char *__restrict__ xs, size_t rhs_bytes,
+                                                    size_t xs_rhs_bytes, int stride, int nel,
                                                     const MxState *__restrict__ st,
                                                     const int *__restrict__ active)
 {
@@ -127,25 +131,32 @@ __global__ __launch_bounds__(kBlock) void k_mx_xpay(char *__restrict__ p, float 
         return;
     const int par = e / stride, i = e - par * stride;
     const size_t off = rhs * rhs_bytes + (size_t)par * spinor_chunks(SF) * 16 * stride;
+    const size_t xoff = rhs * xs_rhs_bytes + (size_t)par * spinor_chunks(WD_F32) * 16 * stride;
     const size_t so = ((size_t)rhs * 2 + par) * stride;
     const float beta = (float)st[rhs].beta;
-    cpx<float> vp[4][3], vr[4][3];
+    const float alpha = (float)st[rhs].alpha;
+    cpx<float> vp[4][3], vr[4][3], vx[4][3];
     Spinor<SF>::template load<float, kStreamNT>(p + off, ps ? ps + so : nullptr, stride, (unsigned)i, vp);
     Spinor<SF>::template load<float, kStreamNT>(r + off, rs ? rs + so : nullptr, stride, (unsigned)i, vr);
+    Spinor<WD_F32>::template load<float, kStreamNT>(xs + xoff, nullptr, stride, (unsigned)i, vx);
 #pragma unroll
     for (int s = 0; s < 4; ++s)
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
+        for (int c = 0; c < 3; ++c) {
+            vx[s][c] = {fmaf(alpha, vp[s][c].r, vx[s][c].r), fmaf(alpha, vp[s][c].i, vx[s][c].i)};
             vp[s][c] = {fmaf(beta, vp[s][c].r, vr[s][c].r), fmaf(beta, vp[s][c].i, vr[s][c].i)};
+        }
+    Spinor<WD_F32>::template store<float, kStreamNT>(xs + xoff, nullptr, stride, (unsigned)i, vx);
     Spinor<SF>::template store<float, kStreamNT>(p + off, ps ? ps + so : nullptr, stride, (unsigned)i,
                                             vp);
 }
 
+// r -= alpha q and the partials of |r|^2; x_s += alpha p is left to the next
+// k_mx_xpay (or the reliable update's k_mx_relx), which read p anyway
 template <int SF>
 __global__ __launch_bounds__(kBlock) void k_mx_update(
-    char *__restrict__ xs, char *__restrict__ r, float *__restrict__ rs,
-    const char *__restrict__ p, const float *__restrict__ ps, const char *__restrict__ q,
-    const float *__restrict__ qs, size_t rhs_bytes, size_t xs_rhs_bytes, int stride, int nel,
+    char *__restrict__ r, float *__restrict__ rs, const char *__restrict__ q,
+    const float *__restrict__ qs, size_t rhs_bytes, int stride, int nel,
     const MxState *__restrict__ st, const int *__restrict__ active, double2 *__restrict__ partials)
 {
     const int rhs = blockIdx.y;
@@ -156,27 +167,19 @@ __global__ __launch_bounds__(kBlock) void k_mx_update(
     if (e < nel) {
         const int par = e / stride, i = e - par * stride;
         const size_t off = rhs * rhs_bytes + (size_t)par * spinor_chunks(SF) * 16 * stride;
-        const size_t xoff = rhs * xs_rhs_bytes + (size_t)par * spinor_chunks(WD_F32) * 16 * stride;
         const size_t so = ((size_t)rhs * 2 + par) * stride;
         const float alpha = (float)st[rhs].alpha;
-        cpx<float> vx[4][3], vr[4][3], vp[4][3], vq[4][3];
-        Spinor<WD_F32>::template load<float, kStreamNT>(xs + xoff, nullptr, stride, (unsigned)i, vx);
+        cpx<float> vr[4][3], vq[4][3];
         Spinor<SF>::template load<float, kStreamNT>(r + off, rs ? rs + so : nullptr, stride,
                                                (unsigned)i, vr);
-        Spinor<SF>::template load<float, kStreamNT>(p + off, ps ? ps + so : nullptr, stride,
-                                               (unsigned)i, vp);
         Spinor<SF>::template load<float, kStreamNT>(q + off, qs ? qs + so : nullptr, stride,
                                                (unsigned)i, vq);
 #pragma unroll
         for (int s = 0; s < 4; ++s)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                vx[s][c] = {fmaf(alpha, vp[s][c].r, vx[s][c].r),
-                            fmaf(alpha, vp[s][c].i, vx[s][c].i)};
+            for (int c = 0; c < 3; ++c)
                 vr[s][c] = {fmaf(-alpha, vq[s][c].r, vr[s][c].r),
                             fmaf(-alpha, vq[s][c].i, vr[s][c].i)};
-            }
-        Spinor<WD_F32>::template store<float, kStreamNT>(xs + xoff, nullptr, stride, (unsigned)i, vx);
         // |r_s|^2 of the stored (rounded) residual
         if constexpr (SF == WD_F16) {
             unsigned w[12];
@@ -197,12 +200,18 @@ __global__ __launch_bounds__(kBlock) void k_mx_update(
     block_reduce_store(acc, 0.0, partials + (size_t)rhs * gridDim.x + blockIdx.x);
 }
 
-// x (F64) += x_s (F32); x_s = 0.  omega: the sloppy frame (x += Omega^dag x_s)
+// x (F64) += x_s (F32); x_s = 0.  omega: the sloppy frame (x += Omega^dag x_s).
+// p != nullptr (mixed CG): x_s first takes the pending step alpha p of the
+// system's last iteration (k_mx_xpay's deferred update, same operation).
+template <int SF>
 __global__ __launch_bounds__(kBlock) void k_mx_relx(double2 *__restrict__ x,
                                                     char *__restrict__ xs, size_t x_rhs_elems,
                                                     size_t xs_rhs_bytes, int stride, int nel,
                                                     const int *__restrict__ reliable,
-                                                    const double2 *__restrict__ omega)
+                                                    const double2 *__restrict__ omega,
+                                                    const char *__restrict__ p,
+                                                    const float *__restrict__ ps, size_t rhs_bytes,
+                                                    const MxState *__restrict__ st)
 {
     const int rhs = blockIdx.y;
     if (!reliable[rhs])
@@ -217,6 +226,19 @@ __global__ __launch_bounds__(kBlock) void k_mx_relx(double2 *__restrict__ x,
     cpx<float> vs[4][3], z[4][3];
     Spinor<WD_F64>::load(xp, nullptr, stride, (unsigned)i, vx);
     Spinor<WD_F32>::load(sp, nullptr, stride, (unsigned)i, vs);
+    if (p) {
+        const float alpha = (float)st[rhs].alpha;
+        const size_t off = rhs * rhs_bytes + (size_t)par * spinor_chunks(SF) * 16 * stride;
+        const size_t so = ((size_t)rhs * 2 + par) * stride;
+        cpx<float> vp[4][3];
+        Spinor<SF>::template load<float, kStreamNT>(p + off, ps ? ps + so : nullptr, stride,
+                                                    (unsigned)i, vp);
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                vs[s][c] = {fmaf(alpha, vp[s][c].r, vs[s][c].r), fmaf(alpha, vp[s][c].i, vs[s][c].i)};
+    }
     omega_rotate(omega, par, i, stride, true, vs);
 #pragma unroll
     for (int s = 0; s < 4; ++s)
@@ -361,6 +383,7 @@ __global__ __launch_bounds__(1024) void k_mx_relfin(const double2 *__restrict__ 
         const double rr = s.x, rsd = sqrt(rr);
         m.true_rsd = rsd;
         m.updates += 1;
+        m.alpha = 0.0; // the pending step went into x with this update
         reliable[r] = 0;
         if (rsd < m.target) {
             m.status = 1;
@@ -1273,13 +1296,13 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
         if (sp == WD_F16)
             hipLaunchKernelGGL(k_mx_xpay<WD_F16>, dim3(eg, n), dim3(kBlock), 0, ctx->stream,
                                (char *)p.f->data, p.f->scale, (const char *)rs.f->data,
-                               (const float *)rs.f->scale, srb, stride, nel, (const MxState *)st,
-                               (const int *)active);
+                               (const float *)rs.f->scale, (char *)xs.f->data, srb, xsb, stride,
+                               nel, (const MxState *)st, (const int *)active);
         else
             hipLaunchKernelGGL(k_mx_xpay<WD_F32>, dim3(eg, n), dim3(kBlock), 0, ctx->stream,
                                (char *)p.f->data, p.f->scale, (const char *)rs.f->data,
-                               (const float *)rs.f->scale, srb, stride, nel, (const MxState *)st,
-                               (const int *)active);
+                               (const float *)rs.f->scale, (char *)xs.f->data, srb, xsb, stride,
+                               nel, (const MxState *)st, (const int *)active);
         if ((e = wd_check_launch(ctx, "k_mx_xpay")))
             return e;
         // (p, q) = (p, A^dag A p) = |A p|^2, summed over the stored A p by the
@@ -1291,16 +1314,14 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
                            (const double2 *)part, nsb, st, active);
         if (sp == WD_F16)
             hipLaunchKernelGGL(k_mx_update<WD_F16>, dim3(eg, n), dim3(kBlock), 0, ctx->stream,
-                               (char *)xs.f->data, (char *)rs.f->data, rs.f->scale,
-                               (const char *)p.f->data, (const float *)p.f->scale,
-                               (const char *)q.f->data, (const float *)q.f->scale, srb, xsb,
-                               stride, nel, (const MxState *)st, (const int *)active, part);
+                               (char *)rs.f->data, rs.f->scale, (const char *)q.f->data,
+                               (const float *)q.f->scale, srb, stride, nel, (const MxState *)st,
+                               (const int *)active, part);
         else
             hipLaunchKernelGGL(k_mx_update<WD_F32>, dim3(eg, n), dim3(kBlock), 0, ctx->stream,
-                               (char *)xs.f->data, (char *)rs.f->data, rs.f->scale,
-                               (const char *)p.f->data, (const float *)p.f->scale,
-                               (const char *)q.f->data, (const float *)q.f->scale, srb, xsb,
-                               stride, nel, (const MxState *)st, (const int *)active, part);
+                               (char *)rs.f->data, rs.f->scale, (const char *)q.f->data,
+                               (const float *)q.f->scale, srb, stride, nel, (const MxState *)st,
+                               (const int *)active, part);
         hipLaunchKernelGGL(k_mx_decide, dim3(n), dim3(1024), 0, ctx->stream,
                            (const double2 *)part, eg, st, active, reliable, prm->max_iter,
                            prm->delta);
@@ -1310,9 +1331,17 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
     // reliable-update slot for the waiting systems
     auto reliable_slot = [&]() -> int {
         int e;
-        hipLaunchKernelGGL(k_mx_relx, dim3(eg, n), dim3(kBlock), 0, ctx->stream, xd,
-                           (char *)xs.f->data, xe, xsb, stride, nel, (const int *)reliable,
-                           (const double2 *)Us->omega);
+        // x += x_s with the pending step alpha p of each due system
+        if (sp == WD_F16)
+            hipLaunchKernelGGL(k_mx_relx<WD_F16>, dim3(eg, n), dim3(kBlock), 0, ctx->stream, xd,
+                               (char *)xs.f->data, xe, xsb, stride, nel, (const int *)reliable,
+                               (const double2 *)Us->omega, (const char *)p.f->data,
+                               (const float *)p.f->scale, srb, (const MxState *)st);
+        else
+            hipLaunchKernelGGL(k_mx_relx<WD_F32>, dim3(eg, n), dim3(kBlock), 0, ctx->stream, xd,
+                               (char *)xs.f->data, xe, xsb, stride, nel, (const int *)reliable,
+                               (const double2 *)Us->omega, (const char *)p.f->data,
+                               (const float *)p.f->scale, srb, (const MxState *)st);
         if ((e = wd_check_launch(ctx, "k_mx_relx")))
             return e;
         if (!eo) {
@@ -1722,9 +1751,10 @@ extern "C" int wd_bicgstab_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, co
     // reliable update of the waiting systems: x += x_s, r = bh - A x (FP64)
     auto reliable_slot = [&]() -> int {
         int e;
-        hipLaunchKernelGGL(k_mx_relx, dim3(eg, n), dim3(kBlock), 0, ctx->stream, xd,
+        hipLaunchKernelGGL(k_mx_relx<WD_F16>, dim3(eg, n), dim3(kBlock), 0, ctx->stream, xd,
                            (char *)xs.f->data, xe, xsb, stride, nel, (const int *)reliable,
-                           (const double2 *)Us->omega);
+                           (const double2 *)Us->omega, (const char *)nullptr,
+                           (const float *)nullptr, (size_t)0, (const MxState *)nullptr);
         if ((e = wd_check_launch(ctx, "k_mx_relx")) ||
             (e = apply_A64(xd, t64d, rel, rel2, stop)))
             return e;
