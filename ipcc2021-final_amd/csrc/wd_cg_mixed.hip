@@ -1624,10 +1624,7 @@ extern "C" int wd_bicgstab_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, co
                                         nullptr, all, n)) ||
         (rc = wd_launch_hop_f64_batch(ctx, (double2 *)Mfull.f->data, bd, Ud, 1, -1, HOP_M, bd,
                                       nullptr, nullptr, nullptr, all, n)) ||
-        (rc = wd_launch_zero(ctx, x->data, x->bytes)) ||
-        (rc = wd_launch_zero(ctx, pv.f->data, pv.f->bytes)) ||
-        (rc = wd_launch_zero(ctx, v.f->data, v.f->bytes)) ||
-        (rc = wd_launch_zero(ctx, xs.f->data, xs.f->bytes)))
+        (rc = wd_launch_zero(ctx, x->data, x->bytes)))
         return rc;
     // r = r0 = bh, rho = (r0, r)
     if (sp == WD_F16)

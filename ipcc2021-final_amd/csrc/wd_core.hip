@@ -355,7 +355,13 @@ static int field_create(wd_ctx *ctx, wd_field **f, int prec, int nrhs, bool work
         wd_field_destroy(x);
         return wd_set_error(ctx, WD_ENOMEM, "field allocation failed");
     }
-    int rc = wd_field_zero(ctx, x);
+    // zeroed in stream order; a user field also waits for it (its caller may
+    // touch it from the host next), a solver's work field does not
+    int rc = wd_launch_zero(ctx, x->data, x->bytes);
+    if (!rc && x->scale)
+        rc = wd_launch_zero(ctx, x->scale, sb);
+    if (!rc && !work && hipStreamSynchronize(ctx->stream) != hipSuccess)
+        rc = wd_set_error(ctx, WD_EHIP, "field zeroing failed");
     if (rc) {
         wd_field_destroy(x);
         return rc;
