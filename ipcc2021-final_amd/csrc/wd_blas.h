@@ -28,6 +28,9 @@ __device__ __forceinline__ void block_reduce_store(double sr, double si, double2
         }
         *dst = make_double2(tr, ti);
     }
+    // lr/li are one array per kernel: a second call must not overwrite them
+    // before thread 0 has read this call's values
+    __syncthreads();
 }
 
 // Sum of `count` partials by one 1024-thread workgroup, fixed order.  The
@@ -60,6 +63,7 @@ __device__ __forceinline__ double2 sum_partials_1024(const double2 *__restrict__
             out.y += li[k];
         }
     }
+    __syncthreads(); // as in block_reduce_store: safe to call repeatedly
     return out;
 }
 

@@ -165,3 +165,34 @@ def test_bicgstab_rhs_per_thread_and_chunk_bitwise(w):
         assert its == out[0][0]
         for r in range(n):
             assert np.array_equal(xs[r], out[0][1][r])
+
+
+def test_bicgstab_repeats_bit_for_bit(w):
+    """Regression: the scalar and elementwise BiCGStab kernels take two block
+    sums each, and the sums share one LDS array per kernel; without a barrier
+    after thread 0 has read the first sum, the second could overwrite it, and
+    repeated solves of one system differed in their last bits (and by an
+    iteration or two).  24^3x48, one system, five solves in one context after
+    a mixed CG solve: identical iterates, iteration counts and residuals."""
+    L = [24, 24, 24, 48]
+    A = w.gauge_near_identity(L, 20211025, 0.3)
+    b = w.spinor_gaussian(L, 20211026, 0)
+    masses = [kappa_to_mass(0.135)]
+    print(f"bicgstab repeat L={L}", flush=True)
+    with w.Context(L, bc=[1, 1, 1, -1]) as ctx:
+        U = ctx.gauge(A)
+        Us = ctx.gauge(A, prec=w.F16, recon=w.RECON_12T)
+        fb = ctx.field(nrhs=1, data=[b])
+        x = ctx.field(nrhs=1)
+        ctx.cg_mixed(x, fb, U, Us, masses, tol=1e-10, delta=1e-2, rhs_per_thread=1,
+                     site_chunk=0, even_odd=True)
+        runs = []
+        for rep in range(5):
+            res = ctx.bicgstab_mixed(x, fb, U, Us, masses, tol=1e-10, delta=1e-2,
+                                     rhs_per_thread=1, site_chunk=0)
+            s = res["systems"][0]
+            runs.append((x.download(0), s["iterations"], s["true_rel_rsd"]))
+            print(rep, s["iterations"], s["true_rel_rsd"], flush=True)
+    assert all(r[1] == runs[0][1] and r[2] == runs[0][2] for r in runs)
+    for r in runs[1:]:
+        assert np.array_equal(r[0], runs[0][0])
