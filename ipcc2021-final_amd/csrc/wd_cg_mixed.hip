@@ -16,16 +16,17 @@
 // without a pending reliable update are skipped by whole workgroups, so the
 // host only polls the flags every `check_every` iterations and the sequence
 // can be replayed from a captured hipGraph.  An iteration:
-//   xpay    p = r_s + beta p
+//   xpay    p = r_s + beta p                (into the next of npend direction buffers)
 //   M       qq = M p, partial |qq|^2       (sloppy, all iterating systems in one launch)
 //   M^dag   q = M^dag qq                   (sloppy)
 //   alpha   alpha = rho / |qq|^2            ((p, M^dag M p) = |M p|^2, as in CGLS)
-//   update  x_s += alpha p, r_s -= alpha q, partial |r_s|^2
+//   update  r_s -= alpha q, partial |r_s|^2 (x_s += alpha p is left pending)
 //   decide  reliable update due?  then the system waits (active = 2), else
 //           beta = |r_s|^2 / rho, rho = |r_s|^2
 // and every `reliable_every` iterations (and at the end of each block) a
 // reliable-update slot:
-//   relx    x += x_s, x_s = 0                             (waiting systems)
+//   flush   x_s += the pending alpha_j p_j (all systems); then
+//           x += x_s, x_s = 0                             (waiting systems)
 //   M, M^dag (FP64) on x                                   (waiting systems)
 //   relr    r = M^dag b - M^dag M x -> r_s, partial |r|^2 (waiting systems)
 //   relfin  converged?  else beta = |r|^2 / rho, rho = |r|^2, iterate again
@@ -33,7 +34,8 @@
 // A waiting system skips the iterations up to the slot, so its iterates and
 // iteration count are those of an update in the same iteration; the slots
 // only save the launches (each a full grid of skipped workgroups) that an
-// update check after every iteration would cost.
+// update check after every iteration would cost.  Groups longer than kPend
+// iterations (reliable_every > kPend) also end with a flush.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -92,6 +94,12 @@ __device__ __forceinline__ void omega_rotate(const double2 *__restrict__ omega, 
 // non-temporal loads and stores keep them from churning L2.
 constexpr bool kStreamNT = true;
 
+// Mixed CG keeps the search directions of up to kPend consecutive iterations
+// in separate buffers and adds their steps alpha_j p_j to x_s in one pass at
+// the end of each group (a reliable-update slot or kPend iterations): x_s is
+// read and written once per group instead of once per iteration.
+constexpr int kPend = 8;
+
 struct MxState {
     double rho;      // |r|^2 carried by the recursion
     double alpha, beta;
@@ -102,7 +110,10 @@ struct MxState {
     double b_norm;   // |M^dag b| (even-odd: of the Schur system, |A^dag bh|)
     double full_b;   // even-odd: |M^dag b| of the full system
     double full_rsd; // even-odd: |M^dag b - M^dag M x| at the last full check
-    int iter, status, updates, pad; // status: 0 running, 1 converged, 2 breakdown, 3 limit
+    int iter, status, updates; // status: 0 running, 1 converged, 2 breakdown, 3 limit
+    int pbuf;                  // direction buffer holding the latest p
+    unsigned pmask;            // bit j: x_s += ahist[j] p_j still pending
+    float ahist[kPend];        // alpha of the pending steps, by buffer
 };
 
 __device__ __forceinline__ double2 sum_rhs_partials(const double2 *partials, int count)
@@ -111,15 +122,16 @@ __device__ __forceinline__ double2 sum_rhs_partials(const double2 *partials, int
 }
 
 // ---- elementwise kernels, blockIdx.y = rhs, one thread per (parity, site) ----
-// x_s += alpha p (the previous iteration's step, deferred from its update:
-// this pass reads p anyway), then p = r + beta p.  alpha is 0 when no step is
-// pending (first iteration, after a reliable update, which applies it).
+// p_pos = r + beta p, p read from the buffer of the system's latest direction
+// (st.pbuf; the same buffer when pbuf == pos: each thread reads its element
+// before writing it).  Buffer j of the directions starts at pb + j * pbytes
+// (scales at ps + j * pscale).
 template <int SF>
-__global__ __launch_bounds__(kBlock) void k_mx_xpay(char *__restrict__ p, float *__restrict__ ps,
+__global__ __launch_bounds__(kBlock) void k_mx_xpay(char *pb, float *ps, size_t pbytes,
+                                                    size_t pscale, int pos,
                                                     const char *__restrict__ r,
                                                     const float *__restrict__ rs,
-                                                    char *__restrict__ xs, size_t rhs_bytes,
-                                                    size_t xs_rhs_bytes, int stride, int nel,
+                                                    size_t rhs_bytes, int stride, int nel,
                                                     const MxState *__restrict__ st,
                                                     const int *__restrict__ active)
 {
@@ -131,28 +143,26 @@ __global__ __launch_bounds__(kBlock) void k_mx_xpay(char *__restrict__ p, float 
         return;
     const int par = e / stride, i = e - par * stride;
     const size_t off = rhs * rhs_bytes + (size_t)par * spinor_chunks(SF) * 16 * stride;
-    const size_t xoff = rhs * xs_rhs_bytes + (size_t)par * spinor_chunks(WD_F32) * 16 * stride;
     const size_t so = ((size_t)rhs * 2 + par) * stride;
+    const int src = st[rhs].pbuf;
     const float beta = (float)st[rhs].beta;
-    const float alpha = (float)st[rhs].alpha;
-    cpx<float> vp[4][3], vr[4][3], vx[4][3];
-    Spinor<SF>::template load<float, kStreamNT>(p + off, ps ? ps + so : nullptr, stride, (unsigned)i, vp);
+    cpx<float> vp[4][3], vr[4][3];
+    Spinor<SF>::template load<float, kStreamNT>(pb + src * pbytes + off,
+                                                ps ? ps + src * pscale + so : nullptr, stride,
+                                                (unsigned)i, vp);
     Spinor<SF>::template load<float, kStreamNT>(r + off, rs ? rs + so : nullptr, stride, (unsigned)i, vr);
-    Spinor<WD_F32>::template load<float, kStreamNT>(xs + xoff, nullptr, stride, (unsigned)i, vx);
 #pragma unroll
     for (int s = 0; s < 4; ++s)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            vx[s][c] = {fmaf(alpha, vp[s][c].r, vx[s][c].r), fmaf(alpha, vp[s][c].i, vx[s][c].i)};
+        for (int c = 0; c < 3; ++c)
             vp[s][c] = {fmaf(beta, vp[s][c].r, vr[s][c].r), fmaf(beta, vp[s][c].i, vr[s][c].i)};
-        }
-    Spinor<WD_F32>::template store<float, kStreamNT>(xs + xoff, nullptr, stride, (unsigned)i, vx);
-    Spinor<SF>::template store<float, kStreamNT>(p + off, ps ? ps + so : nullptr, stride, (unsigned)i,
-                                            vp);
+    Spinor<SF>::template store<float, kStreamNT>(pb + pos * pbytes + off,
+                                                 ps ? ps + pos * pscale + so : nullptr, stride,
+                                                 (unsigned)i, vp);
 }
 
-// r -= alpha q and the partials of |r|^2; x_s += alpha p is left to the next
-// k_mx_xpay (or the reliable update's k_mx_relx), which read p anyway
+// r -= alpha q and the partials of |r|^2; x_s += alpha p waits for the
+// group's k_mx_flush
 template <int SF>
 __global__ __launch_bounds__(kBlock) void k_mx_update(
     char *__restrict__ r, float *__restrict__ rs, const char *__restrict__ q,
@@ -200,45 +210,15 @@ __global__ __launch_bounds__(kBlock) void k_mx_update(
     block_reduce_store(acc, 0.0, partials + (size_t)rhs * gridDim.x + blockIdx.x);
 }
 
-// x (F64) += x_s (F32); x_s = 0.  omega: the sloppy frame (x += Omega^dag x_s).
-// p != nullptr (mixed CG): x_s first takes the pending step alpha p of the
-// system's last iteration (k_mx_xpay's deferred update, same operation).
-template <int SF>
-__global__ __launch_bounds__(kBlock) void k_mx_relx(double2 *__restrict__ x,
-                                                    char *__restrict__ xs, size_t x_rhs_elems,
-                                                    size_t xs_rhs_bytes, int stride, int nel,
-                                                    const int *__restrict__ reliable,
-                                                    const double2 *__restrict__ omega,
-                                                    const char *__restrict__ p,
-                                                    const float *__restrict__ ps, size_t rhs_bytes,
-                                                    const MxState *__restrict__ st)
+// x (F64) of site i, parity par += Omega^dag v (v: x_s in the sloppy frame,
+// omega == nullptr: the identity); x_s = 0
+__device__ __forceinline__ void fold_into_x(double2 *__restrict__ xp, char *__restrict__ sp,
+                                            const double2 *__restrict__ omega, int par, int i,
+                                            int stride, cpx<float> (&vs)[4][3])
 {
-    const int rhs = blockIdx.y;
-    if (!reliable[rhs])
-        return;
-    const int e = blockIdx.x * kBlock + threadIdx.x;
-    if (e >= nel)
-        return;
-    const int par = e / stride, i = e - par * stride;
-    double2 *xp = x + rhs * x_rhs_elems + (size_t)par * 12 * stride;
-    char *sp = xs + rhs * xs_rhs_bytes + (size_t)par * spinor_chunks(WD_F32) * 16 * stride;
     cpx<double> vx[4][3];
-    cpx<float> vs[4][3], z[4][3];
+    cpx<float> z[4][3];
     Spinor<WD_F64>::load(xp, nullptr, stride, (unsigned)i, vx);
-    Spinor<WD_F32>::load(sp, nullptr, stride, (unsigned)i, vs);
-    if (p) {
-        const float alpha = (float)st[rhs].alpha;
-        const size_t off = rhs * rhs_bytes + (size_t)par * spinor_chunks(SF) * 16 * stride;
-        const size_t so = ((size_t)rhs * 2 + par) * stride;
-        cpx<float> vp[4][3];
-        Spinor<SF>::template load<float, kStreamNT>(p + off, ps ? ps + so : nullptr, stride,
-                                                    (unsigned)i, vp);
-#pragma unroll
-        for (int s = 0; s < 4; ++s)
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-                vs[s][c] = {fmaf(alpha, vp[s][c].r, vs[s][c].r), fmaf(alpha, vp[s][c].i, vs[s][c].i)};
-    }
     omega_rotate(omega, par, i, stride, true, vs);
 #pragma unroll
     for (int s = 0; s < 4; ++s)
@@ -250,6 +230,77 @@ __global__ __launch_bounds__(kBlock) void k_mx_relx(double2 *__restrict__ x,
         }
     Spinor<WD_F64>::store(xp, nullptr, stride, (unsigned)i, vx);
     Spinor<WD_F32>::store(sp, nullptr, stride, (unsigned)i, z);
+}
+
+// x (F64) += x_s (F32); x_s = 0 for the systems due (BiCGStab's reliable
+// update).  omega: the sloppy frame (x += Omega^dag x_s).
+__global__ __launch_bounds__(kBlock) void k_mx_relx(double2 *__restrict__ x,
+                                                    char *__restrict__ xs, size_t x_rhs_elems,
+                                                    size_t xs_rhs_bytes, int stride, int nel,
+                                                    const int *__restrict__ reliable,
+                                                    const double2 *__restrict__ omega)
+{
+    const int rhs = blockIdx.y;
+    if (!reliable[rhs])
+        return;
+    const int e = blockIdx.x * kBlock + threadIdx.x;
+    if (e >= nel)
+        return;
+    const int par = e / stride, i = e - par * stride;
+    char *sp = xs + rhs * xs_rhs_bytes + (size_t)par * spinor_chunks(WD_F32) * 16 * stride;
+    cpx<float> vs[4][3];
+    Spinor<WD_F32>::load(sp, nullptr, stride, (unsigned)i, vs);
+    fold_into_x(x + rhs * x_rhs_elems + (size_t)par * 12 * stride, sp, omega, par, i, stride, vs);
+}
+
+// Mixed CG, end of a group of iterations: x_s += alpha_j p_j for the pending
+// steps of each system (st.pmask, in buffer order = iteration order, the same
+// fmaf sequence as one update per iteration); then, for the systems due
+// (reliable[rhs]; reliable == nullptr: none), x += Omega^dag x_s and x_s = 0.
+template <int SF>
+__global__ __launch_bounds__(kBlock) void k_mx_flush(double2 *__restrict__ x,
+                                                     char *__restrict__ xs, size_t x_rhs_elems,
+                                                     size_t xs_rhs_bytes, int stride, int nel,
+                                                     const int *__restrict__ reliable,
+                                                     const double2 *__restrict__ omega,
+                                                     const char *__restrict__ pb,
+                                                     const float *__restrict__ ps, size_t pbytes,
+                                                     size_t pscale, size_t rhs_bytes,
+                                                     const MxState *__restrict__ st)
+{
+    const int rhs = blockIdx.y;
+    const bool due = reliable && reliable[rhs];
+    const unsigned pm = st[rhs].pmask;
+    if (!due && !pm)
+        return;
+    const int e = blockIdx.x * kBlock + threadIdx.x;
+    if (e >= nel)
+        return;
+    const int par = e / stride, i = e - par * stride;
+    char *sp = xs + rhs * xs_rhs_bytes + (size_t)par * spinor_chunks(WD_F32) * 16 * stride;
+    const size_t off = rhs * rhs_bytes + (size_t)par * spinor_chunks(SF) * 16 * stride;
+    const size_t so = ((size_t)rhs * 2 + par) * stride;
+    cpx<float> vs[4][3];
+    Spinor<WD_F32>::template load<float, kStreamNT>(sp, nullptr, stride, (unsigned)i, vs);
+    for (int j = 0; j < kPend; ++j) {
+        if (!((pm >> j) & 1u))
+            continue;
+        const float alpha = st[rhs].ahist[j];
+        cpx<float> vp[4][3];
+        Spinor<SF>::template load<float, kStreamNT>(pb + j * pbytes + off,
+                                                    ps ? ps + j * pscale + so : nullptr, stride,
+                                                    (unsigned)i, vp);
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                vs[s][c] = {fmaf(alpha, vp[s][c].r, vs[s][c].r), fmaf(alpha, vp[s][c].i, vs[s][c].i)};
+    }
+    if (due)
+        fold_into_x(x + rhs * x_rhs_elems + (size_t)par * 12 * stride, sp, omega, par, i, stride,
+                    vs);
+    else
+        Spinor<WD_F32>::template store<float, kStreamNT>(sp, nullptr, stride, (unsigned)i, vs);
 }
 
 // r = bp - q64 (F64) -> r_s (SF, rotated into the sloppy frame by omega when
@@ -326,21 +377,30 @@ __global__ __launch_bounds__(1024) void k_mx_init(const double2 *__restrict__ pa
     }
 }
 
+// alpha of the iteration whose direction went into buffer pos; the step is
+// recorded as pending for the group's k_mx_flush (pos == 0 opens a group:
+// every system's pending set was flushed at the end of the last one)
 __global__ __launch_bounds__(1024) void k_mx_alpha(const double2 *__restrict__ partials, int count,
                                                    MxState *__restrict__ st,
-                                                   int *__restrict__ active)
+                                                   int *__restrict__ active, int pos)
 {
     const int r = blockIdx.x;
+    if (pos == 0 && threadIdx.x == 0)
+        st[r].pmask = 0u;
     if (active[r] != 1)
         return;
     const double2 pq = sum_rhs_partials(partials, count);
     if (threadIdx.x == 0) {
+        MxState &m = st[r];
         if (!isfinite(pq.x) || !(pq.x > 0.0)) {
-            st[r].status = 2;
+            m.status = 2;
             active[r] = 0;
             return;
         }
-        st[r].alpha = st[r].rho / pq.x;
+        m.alpha = m.rho / pq.x;
+        m.ahist[pos] = (float)m.alpha;
+        m.pmask |= 1u << pos;
+        m.pbuf = pos;
     }
 }
 
@@ -383,7 +443,6 @@ __global__ __launch_bounds__(1024) void k_mx_relfin(const double2 *__restrict__ 
         const double rr = s.x, rsd = sqrt(rr);
         m.true_rsd = rsd;
         m.updates += 1;
-        m.alpha = 0.0; // the pending step went into x with this update
         reliable[r] = 0;
         if (rsd < m.target) {
             m.status = 1;
@@ -1117,13 +1176,21 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
     int rc;
 
     const bool eo = prm->even_odd != 0;
+    // automatic spacing of the reliable-update slots (measured on MI355X): 8
+    // for one system, 4 for many, where updates of different systems share
+    // the FP64 launches of a slot
+    const int slot_every = prm->reliable_every > 0 ? prm->reliable_every : (n == 1 ? 8 : 4);
+    // iterations per group of pending x_s steps: one search-direction buffer each
+    const int npend = std::min(slot_every, kPend);
     // device work vectors (even-odd: the Schur system lives on the even half;
-    // the odd half of an operator's output buffer holds its intermediate hop)
+    // the odd half of an operator's output buffer holds its intermediate hop);
+    // p holds npend buffers of n systems
     FieldBuf Mdb, Mfull, t64, q64, rs, p, q, qq, xs;
     if ((rc = wd_field_create_work(ctx, &Mdb.f, WD_F64, n)) || (rc = wd_field_create_work(ctx, &t64.f, WD_F64, n)) ||
         (rc = wd_field_create_work(ctx, &q64.f, WD_F64, n)) || (rc = wd_field_create_work(ctx, &rs.f, sp, n)) ||
-        (rc = wd_field_create_work(ctx, &p.f, sp, n)) || (rc = wd_field_create_work(ctx, &q.f, sp, n)) ||
-        (rc = wd_field_create_work(ctx, &qq.f, sp, n)) || (rc = wd_field_create_work(ctx, &xs.f, WD_F32, n)) ||
+        (rc = wd_field_create_work(ctx, &p.f, sp, npend * n)) ||
+        (rc = wd_field_create_work(ctx, &q.f, sp, n)) || (rc = wd_field_create_work(ctx, &qq.f, sp, n)) ||
+        (rc = wd_field_create_work(ctx, &xs.f, WD_F32, n)) ||
         (eo && (rc = wd_field_create_work(ctx, &Mfull.f, WD_F64, n))))
         return rc;
     const int eg_full = (2 * stride + kBlock - 1) / kBlock; // elementwise workgroups per rhs
@@ -1291,27 +1358,39 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
         return wd_launch_hop_sloppy(ctx, m, sp, Us->prec, Us->recon, R, dag, 0, mode);
     };
 
-    auto iteration = [&]() -> int {
+    // search-direction buffer j: n systems at pbytes * j (scales at pscale * j)
+    const size_t pbytes = (size_t)n * srb, pscale = (size_t)n * 2 * stride;
+    auto pbuf = [&](int j) -> wd_field {
+        wd_field v = *p.f;
+        v.nrhs = n;
+        v.data = (char *)p.f->data + j * pbytes;
+        v.scale = p.f->scale ? p.f->scale + j * pscale : nullptr;
+        return v;
+    };
+
+    // iteration whose new direction goes into buffer pos
+    auto iteration = [&](int pos) -> int {
         int e;
         if (sp == WD_F16)
             hipLaunchKernelGGL(k_mx_xpay<WD_F16>, dim3(eg, n), dim3(kBlock), 0, ctx->stream,
-                               (char *)p.f->data, p.f->scale, (const char *)rs.f->data,
-                               (const float *)rs.f->scale, (char *)xs.f->data, srb, xsb, stride,
+                               (char *)p.f->data, p.f->scale, pbytes, pscale, pos,
+                               (const char *)rs.f->data, (const float *)rs.f->scale, srb, stride,
                                nel, (const MxState *)st, (const int *)active);
         else
             hipLaunchKernelGGL(k_mx_xpay<WD_F32>, dim3(eg, n), dim3(kBlock), 0, ctx->stream,
-                               (char *)p.f->data, p.f->scale, (const char *)rs.f->data,
-                               (const float *)rs.f->scale, (char *)xs.f->data, srb, xsb, stride,
+                               (char *)p.f->data, p.f->scale, pbytes, pscale, pos,
+                               (const char *)rs.f->data, (const float *)rs.f->scale, srb, stride,
                                nel, (const MxState *)st, (const int *)active);
         if ((e = wd_check_launch(ctx, "k_mx_xpay")))
             return e;
         // (p, q) = (p, A^dag A p) = |A p|^2, summed over the stored A p by the
         // launch that writes it, so the A^dag launch reads no third field
-        if ((e = sloppy_op(0, qq.f, p.f, HOP_M_NORM, nullptr)) ||
+        const wd_field pp = pbuf(pos);
+        if ((e = sloppy_op(0, qq.f, &pp, HOP_M_NORM, nullptr)) ||
             (e = sloppy_op(1, q.f, qq.f, HOP_M, nullptr)))
             return e;
         hipLaunchKernelGGL(k_mx_alpha, dim3(n), dim3(1024), 0, ctx->stream,
-                           (const double2 *)part, nsb, st, active);
+                           (const double2 *)part, nsb, st, active, pos);
         if (sp == WD_F16)
             hipLaunchKernelGGL(k_mx_update<WD_F16>, dim3(eg, n), dim3(kBlock), 0, ctx->stream,
                                (char *)rs.f->data, rs.f->scale, (const char *)q.f->data,
@@ -1328,21 +1407,28 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
         return wd_check_launch(ctx, "mixed cg iteration");
     };
 
+    // x_s += the group's pending steps; x += x_s, x_s = 0 for the systems in
+    // due (nullptr: none)
+    auto flush = [&](const int *due) -> int {
+        if (sp == WD_F16)
+            hipLaunchKernelGGL(k_mx_flush<WD_F16>, dim3(eg, n), dim3(kBlock), 0, ctx->stream, xd,
+                               (char *)xs.f->data, xe, xsb, stride, nel, due,
+                               (const double2 *)Us->omega, (const char *)p.f->data,
+                               (const float *)p.f->scale, pbytes, pscale, srb,
+                               (const MxState *)st);
+        else
+            hipLaunchKernelGGL(k_mx_flush<WD_F32>, dim3(eg, n), dim3(kBlock), 0, ctx->stream, xd,
+                               (char *)xs.f->data, xe, xsb, stride, nel, due,
+                               (const double2 *)Us->omega, (const char *)p.f->data,
+                               (const float *)p.f->scale, pbytes, pscale, srb,
+                               (const MxState *)st);
+        return wd_check_launch(ctx, "k_mx_flush");
+    };
+
     // reliable-update slot for the waiting systems
     auto reliable_slot = [&]() -> int {
         int e;
-        // x += x_s with the pending step alpha p of each due system
-        if (sp == WD_F16)
-            hipLaunchKernelGGL(k_mx_relx<WD_F16>, dim3(eg, n), dim3(kBlock), 0, ctx->stream, xd,
-                               (char *)xs.f->data, xe, xsb, stride, nel, (const int *)reliable,
-                               (const double2 *)Us->omega, (const char *)p.f->data,
-                               (const float *)p.f->scale, srb, (const MxState *)st);
-        else
-            hipLaunchKernelGGL(k_mx_relx<WD_F32>, dim3(eg, n), dim3(kBlock), 0, ctx->stream, xd,
-                               (char *)xs.f->data, xe, xsb, stride, nel, (const int *)reliable,
-                               (const double2 *)Us->omega, (const char *)p.f->data,
-                               (const float *)p.f->scale, srb, (const MxState *)st);
-        if ((e = wd_check_launch(ctx, "k_mx_relx")))
+        if ((e = flush(reliable)))
             return e;
         if (!eo) {
             if ((e = wd_launch_hop_f64_batch(ctx, t64d, xd, Ud, 0, -1, HOP_M, xd, nullptr,
@@ -1375,15 +1461,18 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
     // a block of `chunk` iterations with a reliable-update slot after every
     // `slot_every` of them and after the last
     const int chunk = prm->check_every;
-    // automatic spacing (measured on MI355X): 8 for one system, 4 for many,
-    // where updates of different systems share the FP64 launches of a slot
-    const int slot_every = prm->reliable_every > 0 ? prm->reliable_every : (n == 1 ? 8 : 4);
+    // every group of iterations (at most npend) ends with a flush of the
+    // pending x_s steps, fused into the reliable-update slot when one is due
     auto block = [&]() -> int {
         int e = 0;
-        for (int k = 0; k < chunk && !e; ++k) {
-            e = iteration();
-            if (!e && ((k + 1) % slot_every == 0 || k + 1 == chunk))
+        for (int k = 0, pos = 0; k < chunk && !e; ++k) {
+            e = iteration(pos);
+            const bool slot = (k + 1) % slot_every == 0 || k + 1 == chunk;
+            if (!e && slot)
                 e = reliable_slot();
+            else if (!e && pos == npend - 1)
+                e = flush(nullptr);
+            pos = (slot || pos == npend - 1) ? 0 : pos + 1;
         }
         return e;
     };
@@ -1748,10 +1837,9 @@ extern "C" int wd_bicgstab_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, co
     // reliable update of the waiting systems: x += x_s, r = bh - A x (FP64)
     auto reliable_slot = [&]() -> int {
         int e;
-        hipLaunchKernelGGL(k_mx_relx<WD_F16>, dim3(eg, n), dim3(kBlock), 0, ctx->stream, xd,
+        hipLaunchKernelGGL(k_mx_relx, dim3(eg, n), dim3(kBlock), 0, ctx->stream, xd,
                            (char *)xs.f->data, xe, xsb, stride, nel, (const int *)reliable,
-                           (const double2 *)Us->omega, (const char *)nullptr,
-                           (const float *)nullptr, (size_t)0, (const MxState *)nullptr);
+                           (const double2 *)Us->omega);
         if ((e = wd_check_launch(ctx, "k_mx_relx")) ||
             (e = apply_A64(xd, t64d, rel, rel2, stop)))
             return e;

@@ -172,8 +172,10 @@ def test_reliable_update_slots_do_not_change_iterates(w, even_odd):
     iterating, so the slot spacing changes neither iterates nor counts: four
     systems of different masses (updates in different iterations, two per
     thread so a waiting system shares its workgroups with an iterating one),
-    slots after every iteration, every 3rd, and only at the end of each block
-    of 4 (spacing 5 > block)."""
+    slots after every iteration, every 3rd, only at the end of each block of 4
+    (spacing 5 > block), and every 10th in blocks of 20, where groups of 8
+    iterations (the most search directions kept for one x_s pass) end
+    between the slots."""
     L = [4, 4, 6, 8]
     A, bs = _setup(w, L, 4)
     masses = [kappa_to_mass(k) for k in (0.11, 0.12, 0.13, 0.135)]
@@ -182,17 +184,17 @@ def test_reliable_update_slots_do_not_change_iterates(w, even_odd):
         U = ctx.gauge(A)
         Us = ctx.gauge(A, prec=w.F16, recon=12)
         fb = ctx.field(nrhs=4, data=bs)
-        for K in (1, 3, 5):
+        for K, chunk in ((1, 4), (3, 4), (5, 4), (10, 20)):
             x = ctx.field(nrhs=4)
             res = ctx.cg_mixed(x, fb, U, Us, masses, tol=1e-10, rhs_per_thread=2,
-                               check_every=4, even_odd=even_odd, reliable_every=K)
+                               check_every=chunk, even_odd=even_odd, reliable_every=K)
             out.append(([x.download(r) for r in range(4)], res))
             x.close()
     its = [[s["iterations"] for s in o[1]["systems"]] for o in out]
     ups = [[s["reliable_updates"] for s in o[1]["systems"]] for o in out]
     print("iterations", its, "updates", ups, flush=True)
     assert len(set(its[0])) > 1  # the systems do stop at different iterations
-    for k in (1, 2):
+    for k in (1, 2, 3):
         assert its[k] == its[0] and ups[k] == ups[0]
         assert [s["true_rel_rsd"] for s in out[k][1]["systems"]] == \
             [s["true_rel_rsd"] for s in out[0][1]["systems"]]
