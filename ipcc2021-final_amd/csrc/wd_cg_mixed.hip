@@ -232,27 +232,6 @@ __device__ __forceinline__ void fold_into_x(double2 *__restrict__ xp, char *__re
     Spinor<WD_F32>::store(sp, nullptr, stride, (unsigned)i, z);
 }
 
-// x (F64) += x_s (F32); x_s = 0 for the systems due (BiCGStab's reliable
-// update).  omega: the sloppy frame (x += Omega^dag x_s).
-__global__ __launch_bounds__(kBlock) void k_mx_relx(double2 *__restrict__ x,
-                                                    char *__restrict__ xs, size_t x_rhs_elems,
-                                                    size_t xs_rhs_bytes, int stride, int nel,
-                                                    const int *__restrict__ reliable,
-                                                    const double2 *__restrict__ omega)
-{
-    const int rhs = blockIdx.y;
-    if (!reliable[rhs])
-        return;
-    const int e = blockIdx.x * kBlock + threadIdx.x;
-    if (e >= nel)
-        return;
-    const int par = e / stride, i = e - par * stride;
-    char *sp = xs + rhs * xs_rhs_bytes + (size_t)par * spinor_chunks(WD_F32) * 16 * stride;
-    cpx<float> vs[4][3];
-    Spinor<WD_F32>::load(sp, nullptr, stride, (unsigned)i, vs);
-    fold_into_x(x + rhs * x_rhs_elems + (size_t)par * 12 * stride, sp, omega, par, i, stride, vs);
-}
-
 // Mixed CG, end of a group of iterations: x_s += alpha_j p_j for the pending
 // steps of each system (st.pmask, in buffer order = iteration order, the same
 // fmaf sequence as one update per iteration); then, for the systems due
@@ -578,7 +557,10 @@ struct BsState {
     double2 rho, alpha, omega, beta;
     double2 r0r;     // (r0, r) of the last reliable update (restarts after a full check)
     double maxr, target, rsd, true_rsd, b_norm, full_b, full_rsd;
-    int iter, status, updates, pad; // status: 0 running, 1 converged, 2 breakdown, 3 limit
+    int iter, status, updates; // status: 0 running, 1 converged, 2 breakdown, 3 limit
+    int pbuf;                  // direction buffer holding the latest p
+    unsigned pmask;            // bit j: x_s += alpha_j p_j + omega_j s_j still pending
+    float ahist[kPend][2], whist[kPend][2]; // (re, im) of alpha_j and omega_j, by buffer
 };
 
 __device__ __forceinline__ double2 zmul(double2 a, double2 b)
@@ -601,9 +583,12 @@ __device__ __forceinline__ bool zfinite(double2 a) { return isfinite(a.x) && isf
     const size_t off = rhs * rhs_bytes + (size_t)par * spinor_chunks(SFMT) * 16 * stride;         \
     const size_t so = ((size_t)rhs * 2 + par) * stride;
 
-// p = r + beta (p - omega v)
+// p_pos = r + beta (p - omega v), p read from the buffer of the system's latest
+// direction (st.pbuf; in place when pbuf == pos).  Direction buffer j starts
+// at pb + j * pbytes (scales at ps + j * pscale).
 template <int SF>
-__global__ __launch_bounds__(kBlock) void k_bs_p(char *__restrict__ p, float *__restrict__ ps,
+__global__ __launch_bounds__(kBlock) void k_bs_p(char *pb, float *ps, size_t pbytes,
+                                                 size_t pscale, int pos,
                                                  const char *__restrict__ r,
                                                  const float *__restrict__ rs,
                                                  const char *__restrict__ v,
@@ -614,10 +599,13 @@ __global__ __launch_bounds__(kBlock) void k_bs_p(char *__restrict__ p, float *__
     if (active[blockIdx.y] != 1 || blockIdx.x * kBlock + threadIdx.x >= nel)
         return;
     BS_ELEM_PROLOGUE(SF)
+    const int src = st[rhs].pbuf;
     const float br = (float)st[rhs].beta.x, bi = (float)st[rhs].beta.y;
     const float wr = (float)st[rhs].omega.x, wi = (float)st[rhs].omega.y;
     cpx<float> vp[4][3], vr[4][3], vv[4][3];
-    Spinor<SF>::template load<float, kStreamNT>(p + off, ps ? ps + so : nullptr, stride, (unsigned)i, vp);
+    Spinor<SF>::template load<float, kStreamNT>(pb + src * pbytes + off,
+                                                ps ? ps + src * pscale + so : nullptr, stride,
+                                                (unsigned)i, vp);
     Spinor<SF>::template load<float, kStreamNT>(r + off, rs ? rs + so : nullptr, stride, (unsigned)i, vr);
     Spinor<SF>::template load<float, kStreamNT>(v + off, vs ? vs + so : nullptr, stride, (unsigned)i, vv);
 #pragma unroll
@@ -629,7 +617,9 @@ __global__ __launch_bounds__(kBlock) void k_bs_p(char *__restrict__ p, float *__
             const float ui = vp[s][c].i - (wr * vv[s][c].i + wi * vv[s][c].r);
             vp[s][c] = {vr[s][c].r + (br * ur - bi * ui), vr[s][c].i + (br * ui + bi * ur)};
         }
-    Spinor<SF>::template store<float, kStreamNT>(p + off, ps ? ps + so : nullptr, stride, (unsigned)i, vp);
+    Spinor<SF>::template store<float, kStreamNT>(pb + pos * pbytes + off,
+                                                 ps ? ps + pos * pscale + so : nullptr, stride,
+                                                 (unsigned)i, vp);
 }
 
 // s = r - alpha v
@@ -658,16 +648,31 @@ __global__ __launch_bounds__(kBlock) void k_bs_s(char *__restrict__ sv, float *_
     Spinor<SF>::template store<float, kStreamNT>(sv + off, ss ? ss + so : nullptr, stride, (unsigned)i, vr);
 }
 
-// x_s += alpha p + omega s; r = s - omega t; partials |r|^2 (pa) and (r0, r) (pb)
-// of the stored r
+// x_s += alpha p + omega s (one step; the expression k_bs_flush applies per
+// pending step)
+__device__ __forceinline__ void bs_xstep(cpx<float> (&vx)[4][3], const cpx<float> (&vp)[4][3],
+                                         const cpx<float> (&vs)[4][3], float ar, float ai,
+                                         float wr, float wi)
+{
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const cpx<float> P = vp[s][c], S = vs[s][c];
+            vx[s][c] = {vx[s][c].r + (ar * P.r - ai * P.i) + (wr * S.r - wi * S.i),
+                        vx[s][c].i + (ar * P.i + ai * P.r) + (wr * S.i + wi * S.r)};
+        }
+}
+
+// r = s - omega t; partials |r|^2 (pa) and (r0, r) (pb) of the stored r;
+// x_s += alpha p + omega s is left pending for the group's k_bs_flush
 template <int SF>
 __global__ __launch_bounds__(kBlock) void k_bs_update(
-    char *__restrict__ xs, char *__restrict__ r, float *__restrict__ rs, const char *__restrict__ p,
-    const float *__restrict__ ps, const char *__restrict__ sv, const float *__restrict__ ss,
-    const char *__restrict__ t, const float *__restrict__ ts, const char *__restrict__ r0,
-    const float *__restrict__ r0s, size_t rhs_bytes, size_t xs_rhs_bytes, int stride, int nel,
-    const BsState *__restrict__ st, const int *__restrict__ active, double2 *__restrict__ pa,
-    double2 *__restrict__ pb)
+    char *__restrict__ r, float *__restrict__ rs, const char *__restrict__ sv,
+    const float *__restrict__ ss, const char *__restrict__ t, const float *__restrict__ ts,
+    const char *__restrict__ r0, const float *__restrict__ r0s, size_t rhs_bytes, int stride,
+    int nel, const BsState *__restrict__ st, const int *__restrict__ active,
+    double2 *__restrict__ pa, double2 *__restrict__ pb)
 {
     const int rhs = blockIdx.y;
     if (active[rhs] != 1)
@@ -675,13 +680,8 @@ __global__ __launch_bounds__(kBlock) void k_bs_update(
     double rr = 0.0, dr = 0.0, di = 0.0;
     if (blockIdx.x * kBlock + threadIdx.x < nel) {
         BS_ELEM_PROLOGUE(SF)
-        const size_t xoff = rhs * xs_rhs_bytes + (size_t)par * spinor_chunks(WD_F32) * 16 * stride;
-        const float ar = (float)st[rhs].alpha.x, ai = (float)st[rhs].alpha.y;
         const float wr = (float)st[rhs].omega.x, wi = (float)st[rhs].omega.y;
-        cpx<float> vx[4][3], vp[4][3], vs[4][3], vt[4][3];
-        Spinor<WD_F32>::template load<float, kStreamNT>(xs + xoff, nullptr, stride, (unsigned)i, vx);
-        Spinor<SF>::template load<float, kStreamNT>(p + off, ps ? ps + so : nullptr, stride, (unsigned)i,
-                                               vp);
+        cpx<float> vs[4][3], vt[4][3];
         Spinor<SF>::template load<float, kStreamNT>(sv + off, ss ? ss + so : nullptr, stride,
                                                (unsigned)i, vs);
         Spinor<SF>::template load<float, kStreamNT>(t + off, ts ? ts + so : nullptr, stride, (unsigned)i,
@@ -690,12 +690,9 @@ __global__ __launch_bounds__(kBlock) void k_bs_update(
         for (int s = 0; s < 4; ++s)
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const cpx<float> P = vp[s][c], S = vs[s][c], T = vt[s][c];
-                vx[s][c] = {vx[s][c].r + (ar * P.r - ai * P.i) + (wr * S.r - wi * S.i),
-                            vx[s][c].i + (ar * P.i + ai * P.r) + (wr * S.i + wi * S.r)};
+                const cpx<float> S = vs[s][c], T = vt[s][c];
                 vs[s][c] = {S.r - (wr * T.r - wi * T.i), S.i - (wr * T.i + wi * T.r)};
             }
-        Spinor<WD_F32>::template store<float, kStreamNT>(xs + xoff, nullptr, stride, (unsigned)i, vx);
         cpx<float> vq[4][3]; // r as stored
         if constexpr (SF == WD_F16) {
             unsigned wv[12];
@@ -725,6 +722,49 @@ __global__ __launch_bounds__(kBlock) void k_bs_update(
     }
     block_reduce_store(rr, 0.0, pa + (size_t)rhs * gridDim.x + blockIdx.x);
     block_reduce_store(dr, di, pb + (size_t)rhs * gridDim.x + blockIdx.x);
+}
+
+// End of a group of BiCGStab iterations: x_s += alpha_j p_j + omega_j s_j for
+// the pending steps of each system (st.pmask, in buffer order = iteration
+// order: the arithmetic of one update per iteration); then, for the systems
+// due (reliable[rhs]; reliable == nullptr: none), x += Omega^dag x_s and
+// x_s = 0.  Buffer j of the directions p and of s starts at pb / sb + j * bytes
+// (scales at ps / ss + j * scale).
+template <int SF>
+__global__ __launch_bounds__(kBlock) void k_bs_flush(
+    double2 *__restrict__ x, char *__restrict__ xs, size_t x_rhs_elems, size_t xs_rhs_bytes,
+    int stride, int nel, const int *__restrict__ reliable, const double2 *__restrict__ omega,
+    const char *__restrict__ pb, const float *__restrict__ ps, const char *__restrict__ sb,
+    const float *__restrict__ ss, size_t bytes, size_t scale, size_t rhs_bytes,
+    const BsState *__restrict__ st)
+{
+    const bool due = reliable && reliable[blockIdx.y];
+    const unsigned pm = st[blockIdx.y].pmask;
+    if ((!due && !pm) || blockIdx.x * kBlock + threadIdx.x >= nel)
+        return;
+    BS_ELEM_PROLOGUE(SF)
+    char *sp = xs + rhs * xs_rhs_bytes + (size_t)par * spinor_chunks(WD_F32) * 16 * stride;
+    cpx<float> vx[4][3];
+    Spinor<WD_F32>::template load<float, kStreamNT>(sp, nullptr, stride, (unsigned)i, vx);
+    for (int j = 0; j < kPend; ++j) {
+        if (!((pm >> j) & 1u))
+            continue;
+        const float ar = st[rhs].ahist[j][0], ai = st[rhs].ahist[j][1];
+        const float wr = st[rhs].whist[j][0], wi = st[rhs].whist[j][1];
+        cpx<float> vp[4][3], vs[4][3];
+        Spinor<SF>::template load<float, kStreamNT>(pb + j * bytes + off,
+                                                    ps ? ps + j * scale + so : nullptr, stride,
+                                                    (unsigned)i, vp);
+        Spinor<SF>::template load<float, kStreamNT>(sb + j * bytes + off,
+                                                    ss ? ss + j * scale + so : nullptr, stride,
+                                                    (unsigned)i, vs);
+        bs_xstep(vx, vp, vs, ar, ai, wr, wi);
+    }
+    if (due)
+        fold_into_x(x + rhs * x_rhs_elems + (size_t)par * 12 * stride, sp, omega, par, i, stride,
+                    vx);
+    else
+        Spinor<WD_F32>::template store<float, kStreamNT>(sp, nullptr, stride, (unsigned)i, vx);
 }
 
 // r = bp - q64 (F64; q64 == nullptr: r = bp) -> sloppy r (and r0 = r when r0w);
@@ -819,16 +859,21 @@ __global__ __launch_bounds__(1024) void k_bs_init(const double2 *__restrict__ pa
     }
 }
 
-// alpha = rho / (r0, v)
+// alpha = rho / (r0, v) of the iteration whose direction went into buffer pos
+// (pos == 0 opens a group: every pending step was flushed at the end of the
+// last one)
 __global__ __launch_bounds__(1024) void k_bs_alpha(const double2 *__restrict__ pa, int count,
                                                    BsState *__restrict__ st,
-                                                   int *__restrict__ active)
+                                                   int *__restrict__ active, int pos)
 {
     const int r = blockIdx.x;
+    if (pos == 0 && threadIdx.x == 0)
+        st[r].pmask = 0u;
     if (active[r] != 1)
         return;
     const double2 d = sum_at(pa, count);
     if (threadIdx.x == 0) {
+        st[r].pbuf = pos;
         if (!zfinite(d) || !(d.x * d.x + d.y * d.y > 0.0)) {
             st[r].status = 2;
             active[r] = 0;
@@ -839,11 +884,12 @@ __global__ __launch_bounds__(1024) void k_bs_alpha(const double2 *__restrict__ p
 }
 
 // omega = (t, s) / (t, t) = conj((s, t)) / |t|^2; t = 0 (s = 0: the system is
-// solved) gives omega = 0
+// solved) gives omega = 0.  The step x_s += alpha p + omega s of the iteration
+// (buffer pos) is recorded as pending.
 __global__ __launch_bounds__(1024) void k_bs_omega(const double2 *__restrict__ pst, int cst,
                                                    const double2 *__restrict__ ptt, int ctt,
                                                    BsState *__restrict__ st,
-                                                   int *__restrict__ active)
+                                                   int *__restrict__ active, int pos)
 {
     const int r = blockIdx.x;
     if (active[r] != 1)
@@ -856,8 +902,13 @@ __global__ __launch_bounds__(1024) void k_bs_omega(const double2 *__restrict__ p
             active[r] = 0;
             return;
         }
-        st[r].omega = tt.x > 0.0 ? make_double2(s_t.x / tt.x, -s_t.y / tt.x)
-                                 : make_double2(0.0, 0.0);
+        BsState &m = st[r];
+        m.omega = tt.x > 0.0 ? make_double2(s_t.x / tt.x, -s_t.y / tt.x) : make_double2(0.0, 0.0);
+        m.ahist[pos][0] = (float)m.alpha.x;
+        m.ahist[pos][1] = (float)m.alpha.y;
+        m.whist[pos][0] = (float)m.omega.x;
+        m.whist[pos][1] = (float)m.omega.y;
+        m.pmask |= 1u << pos;
     }
 }
 
@@ -1649,14 +1700,20 @@ extern "C" int wd_bicgstab_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, co
     const int stride = (int)ctx->stride;
     int rc;
 
+    // reliable-update slots every slot_every iterations (8 for one system, 4 for
+    // many, as the CG); groups of npend iterations keep their p and s in
+    // separate buffers for one x_s pass per group
+    const int slot_every = prm->reliable_every > 0 ? prm->reliable_every : (n == 1 ? 8 : 4);
+    const int npend = std::min(slot_every, kPend);
     // F64: bh (even half), A x, M^dag b and the check's M^dag M x; sloppy: r, r0,
-    // p, v, s, t (the odd halves of v and t hold the operator's intermediate hop)
+    // p and s (npend buffers of n systems each), v, t (the odd halves of v and t
+    // hold the operator's intermediate hop)
     FieldBuf bh, Mfull, t64, q64, r, r0, pv, v, sv, t, xs;
     if ((rc = wd_field_create_work(ctx, &bh.f, WD_F64, n)) || (rc = wd_field_create_work(ctx, &t64.f, WD_F64, n)) ||
         (rc = wd_field_create_work(ctx, &q64.f, WD_F64, n)) || (rc = wd_field_create_work(ctx, &Mfull.f, WD_F64, n)) ||
         (rc = wd_field_create_work(ctx, &r.f, sp, n)) || (rc = wd_field_create_work(ctx, &r0.f, sp, n)) ||
-        (rc = wd_field_create_work(ctx, &pv.f, sp, n)) || (rc = wd_field_create_work(ctx, &v.f, sp, n)) ||
-        (rc = wd_field_create_work(ctx, &sv.f, sp, n)) || (rc = wd_field_create_work(ctx, &t.f, sp, n)) ||
+        (rc = wd_field_create_work(ctx, &pv.f, sp, npend * n)) || (rc = wd_field_create_work(ctx, &v.f, sp, n)) ||
+        (rc = wd_field_create_work(ctx, &sv.f, sp, npend * n)) || (rc = wd_field_create_work(ctx, &t.f, sp, n)) ||
         (rc = wd_field_create_work(ctx, &xs.f, WD_F32, n)))
         return rc;
     const int eg_full = (2 * stride + kBlock - 1) / kBlock;
@@ -1807,26 +1864,38 @@ extern "C" int wd_bicgstab_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, co
             hipLaunchKernelGGL(K<WD_F32>, dim3(eg, n), dim3(kBlock), 0, ctx->stream, __VA_ARGS__);  \
     } while (0)
 
-    auto iteration = [&]() -> int {
+    // buffer j of p or s: n systems at bbytes * j (scales at bscale * j)
+    const size_t bbytes = (size_t)n * srb, bscale = (size_t)n * 2 * stride;
+    auto buf = [&](const FieldBuf &f, int j) -> wd_field {
+        wd_field b = *f.f;
+        b.nrhs = n;
+        b.data = (char *)f.f->data + j * bbytes;
+        b.scale = f.f->scale ? f.f->scale + j * bscale : nullptr;
+        return b;
+    };
+
+    // iteration whose p and s go into buffers pos
+    auto iteration = [&](int pos) -> int {
         int e;
-        BS_SF(k_bs_p, (char *)pv.f->data, pv.f->scale, (const char *)r.f->data,
-              (const float *)r.f->scale, (const char *)v.f->data, (const float *)v.f->scale, srb,
-              stride, nel, (const BsState *)st, (const int *)active);
-        if ((e = apply_A(v.f, pv.f, r0.f, false)))
+        const wd_field pp = buf(pv, pos), ss = buf(sv, pos);
+        BS_SF(k_bs_p, (char *)pv.f->data, pv.f->scale, bbytes, bscale, pos,
+              (const char *)r.f->data, (const float *)r.f->scale, (const char *)v.f->data,
+              (const float *)v.f->scale, srb, stride, nel, (const BsState *)st,
+              (const int *)active);
+        if ((e = apply_A(v.f, &pp, r0.f, false)))
             return e;
         hipLaunchKernelGGL(k_bs_alpha, dim3(n), dim3(1024), 0, ctx->stream, (const double2 *)pa,
-                           op_parts, st, active);
-        BS_SF(k_bs_s, (char *)sv.f->data, sv.f->scale, (const char *)r.f->data,
+                           op_parts, st, active, pos);
+        BS_SF(k_bs_s, (char *)ss.data, ss.scale, (const char *)r.f->data,
               (const float *)r.f->scale, (const char *)v.f->data, (const float *)v.f->scale, srb,
               stride, nel, (const BsState *)st, (const int *)active);
-        if ((e = apply_A(t.f, sv.f, sv.f, true)))   // t = A s, (s, t) into pa, |t|^2 into pb
+        if ((e = apply_A(t.f, &ss, &ss, true)))   // t = A s, (s, t) into pa, |t|^2 into pb
             return e;
         hipLaunchKernelGGL(k_bs_omega, dim3(n), dim3(1024), 0, ctx->stream, (const double2 *)pa,
-                           op_parts, (const double2 *)pb, op_parts, st, active);
-        BS_SF(k_bs_update, (char *)xs.f->data, (char *)r.f->data, r.f->scale,
-              (const char *)pv.f->data, (const float *)pv.f->scale, (const char *)sv.f->data,
-              (const float *)sv.f->scale, (const char *)t.f->data, (const float *)t.f->scale,
-              (const char *)r0.f->data, (const float *)r0.f->scale, srb, xsb, stride, nel,
+                           op_parts, (const double2 *)pb, op_parts, st, active, pos);
+        BS_SF(k_bs_update, (char *)r.f->data, r.f->scale, (const char *)ss.data,
+              (const float *)ss.scale, (const char *)t.f->data, (const float *)t.f->scale,
+              (const char *)r0.f->data, (const float *)r0.f->scale, srb, stride, nel,
               (const BsState *)st, (const int *)active, pa, pb);
         hipLaunchKernelGGL(k_bs_decide, dim3(n), dim3(1024), 0, ctx->stream, (const double2 *)pa,
                            (const double2 *)pb, eg, st, active, reliable, prm->max_iter,
@@ -1834,14 +1903,20 @@ extern "C" int wd_bicgstab_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, co
         return wd_check_launch(ctx, "mixed bicgstab iteration");
     };
 
+    // x_s += the group's pending steps; x += x_s, x_s = 0 for the systems in due
+    // (nullptr: none)
+    auto flush = [&](const int *due) -> int {
+        BS_SF(k_bs_flush, xd, (char *)xs.f->data, xe, xsb, stride, nel, due,
+              (const double2 *)Us->omega, (const char *)pv.f->data, (const float *)pv.f->scale,
+              (const char *)sv.f->data, (const float *)sv.f->scale, bbytes, bscale, srb,
+              (const BsState *)st);
+        return wd_check_launch(ctx, "k_bs_flush");
+    };
+
     // reliable update of the waiting systems: x += x_s, r = bh - A x (FP64)
     auto reliable_slot = [&]() -> int {
         int e;
-        hipLaunchKernelGGL(k_mx_relx, dim3(eg, n), dim3(kBlock), 0, ctx->stream, xd,
-                           (char *)xs.f->data, xe, xsb, stride, nel, (const int *)reliable,
-                           (const double2 *)Us->omega);
-        if ((e = wd_check_launch(ctx, "k_mx_relx")) ||
-            (e = apply_A64(xd, t64d, rel, rel2, stop)))
+        if ((e = flush(reliable)) || (e = apply_A64(xd, t64d, rel, rel2, stop)))
             return e;
         BS_SF(k_bs_relr, (char *)r.f->data, r.f->scale, (char *)r0.f->data, r0.f->scale, false,
               (const double2 *)bhd, (const double2 *)t64d, xe, srb, stride, nel,
@@ -1855,13 +1930,18 @@ extern "C" int wd_bicgstab_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, co
 #undef BS_SF
 
     const int chunk = prm->check_every;
-    const int slot_every = prm->reliable_every > 0 ? prm->reliable_every : (n == 1 ? 8 : 4);
+    // every group of iterations (at most npend) ends with a flush of the
+    // pending x_s steps, fused into the reliable-update slot when one is due
     auto block = [&]() -> int {
         int e = 0;
-        for (int k = 0; k < chunk && !e; ++k) {
-            e = iteration();
-            if (!e && ((k + 1) % slot_every == 0 || k + 1 == chunk))
+        for (int k = 0, pos = 0; k < chunk && !e; ++k) {
+            e = iteration(pos);
+            const bool slot = (k + 1) % slot_every == 0 || k + 1 == chunk;
+            if (!e && slot)
                 e = reliable_slot();
+            else if (!e && pos == npend - 1)
+                e = flush(nullptr);
+            pos = (slot || pos == npend - 1) ? 0 : pos + 1;
         }
         return e;
     };

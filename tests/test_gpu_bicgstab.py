@@ -85,7 +85,9 @@ def test_bicgstab_closed_forms(w):
 
 def test_bicgstab_work_orders_bitwise(w):
     """Graph replay or direct launches, reliable-update slots after every
-    iteration or every 3rd: identical iterates and counts."""
+    iteration, every 3rd, or every 10th in blocks of 20 (groups of 8
+    iterations, the most whose p and s are kept for one x_s pass, end between
+    the slots): identical iterates and counts."""
     L = [4, 4, 4, 8]
     A = w.gauge_near_identity(L, 81, 0.3)
     bs = [w.spinor_gaussian(L, 82, r) for r in range(2)]
@@ -95,13 +97,13 @@ def test_bicgstab_work_orders_bitwise(w):
         U = ctx.gauge(A)
         Us = ctx.gauge(A, prec=w.F16, recon=12)
         fb = ctx.field(nrhs=2, data=bs)
-        for g, K in ((True, 1), (False, 1), (True, 3)):
+        for g, K, chunk in ((True, 1, 3), (False, 1, 3), (True, 3, 3), (True, 10, 20)):
             X = ctx.field(nrhs=2)
-            res = ctx.bicgstab_mixed(X, fb, U, Us, masses, use_graph=g, check_every=3,
+            res = ctx.bicgstab_mixed(X, fb, U, Us, masses, use_graph=g, check_every=chunk,
                                      rhs_per_thread=2, reliable_every=K)
             out.append(([X.download(r) for r in range(2)], res))
             X.close()
-    for k in (1, 2):
+    for k in (1, 2, 3):
         for r in range(2):
             assert np.array_equal(out[k][0][r], out[0][0][r])
             assert out[k][1]["systems"][r]["iterations"] == out[0][1]["systems"][r]["iterations"]
