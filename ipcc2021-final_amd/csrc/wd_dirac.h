@@ -25,13 +25,6 @@ __device__ __forceinline__ cpx<T> cadd(cpx<T> a, cpx<T> b)
     return {a.r + b.r, a.i + b.i};
 }
 
-// a * b
-template <typename T>
-__device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b)
-{
-    return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r};
-}
-
 // acc + a * b
 template <typename T>
 __device__ __forceinline__ cpx<T> cfma(cpx<T> acc, cpx<T> a, cpx<T> b)
@@ -155,11 +148,12 @@ __device__ __forceinline__ void reconstruct_row2(cpx<T> (&U)[3][3])
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const int a = (c + 1) % 3, b = (c + 2) % 3;
-        // (row0 x row1)_c = row0_a row1_b - row0_b row1_a
-        cpx<T> x = cmul(U[0][a], U[1][b]);
-        const cpx<T> y = cmul(U[0][b], U[1][a]);
-        x.r -= y.r;
-        x.i -= y.i;
-        U[2][c] = {x.r, -x.i};
+        // (row0 x row1)_c = row0_a row1_b - row0_b row1_a, each complex product
+        // with explicit FMAs: the rounding is then the same in every kernel
+        // that rebuilds row 2, whatever products the compiler would contract
+        const cpx<T> p = U[0][a], q = U[1][b], s = U[0][b], t = U[1][a];
+        const T xr = fma(p.r, q.r, -(p.i * q.i)), xi = fma(p.r, q.i, p.i * q.r);
+        const T yr = fma(s.r, t.r, -(s.i * t.i)), yi = fma(s.r, t.i, s.i * t.r);
+        U[2][c] = {xr - yr, yi - xi};
     }
 }

@@ -91,6 +91,17 @@ __device__ __forceinline__ unsigned f2h(float re, float im)
     return *reinterpret_cast<unsigned *>(&h);
 }
 
+// f2h(re * s, im * s) with the products rounded to FP32 first, in every
+// kernel: the compiler may otherwise fuse a product and its F16 conversion
+// into one mixed-precision FMA (one rounding instead of two) in some kernels
+// and not in others, and two kernels packing the same values would disagree.
+__device__ __forceinline__ unsigned f2h_scaled(float re, float im, float s)
+{
+    float a = re * s, b = im * s;
+    asm volatile("" : "+v"(a), "+v"(b));
+    return f2h(a, b);
+}
+
 // ---- spinor sites ---------------------------------------------------------------
 // base points at the parity block of one rhs; scale at that parity's scales (F16).
 template <int FMT>
@@ -209,7 +220,7 @@ struct Spinor<WD_F16> {
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const cpx<R> z = v[(4 * j + q) / 3][(4 * j + q) % 3];
-                wout[4 * j + q] = f2h((float)z.r * inv, (float)z.i * inv);
+                wout[4 * j + q] = f2h_scaled((float)z.r, (float)z.i, inv);
             }
             u4v c;
             c.x = wout[4 * j];

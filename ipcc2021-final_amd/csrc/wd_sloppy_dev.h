@@ -174,7 +174,7 @@ __device__ __forceinline__ float pack16(const cpx<float> (&v)[4][3], unsigned (&
     const float inv = tiny ? 0.f : 1.0f / m;
 #pragma unroll
     for (int k = 0; k < 12; ++k)
-        w[k] = f2h(v[k / 3][k % 3].r * inv, v[k / 3][k % 3].i * inv);
+        w[k] = f2h_scaled(v[k / 3][k % 3].r, v[k / 3][k % 3].i, inv);
     return tiny ? 0.f : m;
 }
 
