@@ -18,9 +18,10 @@
 // can be replayed from a captured hipGraph.  An iteration:
 //   xpay    p = r_s + beta p                (into the next of npend direction buffers)
 //   M       qq = M p, partial |qq|^2       (sloppy, all iterating systems in one launch)
-//   M^dag   q = M^dag qq                   (sloppy)
 //   alpha   alpha = rho / |qq|^2            ((p, M^dag M p) = |M p|^2, as in CGLS)
-//   update  r_s -= alpha q, partial |r_s|^2 (x_s += alpha p is left pending)
+//   M^dag   r_s -= alpha M^dag qq, partial |r_s|^2 (sloppy, HOP_M_AXPY: the
+//           update in the operator launch, q = M^dag qq is never stored;
+//           x_s += alpha p is left pending)
 //   decide  reliable update due?  then the system waits (active = 2), else
 //           beta = |r_s|^2 / rho, rho = |r_s|^2
 // and every `reliable_every` iterations (and at the end of each block) a
@@ -159,55 +160,6 @@ __global__ __launch_bounds__(kBlock) void k_mx_xpay(char *pb, float *ps, size_t 
     Spinor<SF>::template store<float, kStreamNT>(pb + pos * pbytes + off,
                                                  ps ? ps + pos * pscale + so : nullptr, stride,
                                                  (unsigned)i, vp);
-}
-
-// r -= alpha q and the partials of |r|^2; x_s += alpha p waits for the
-// group's k_mx_flush
-template <int SF>
-__global__ __launch_bounds__(kBlock) void k_mx_update(
-    char *__restrict__ r, float *__restrict__ rs, const char *__restrict__ q,
-    const float *__restrict__ qs, size_t rhs_bytes, int stride, int nel,
-    const MxState *__restrict__ st, const int *__restrict__ active, double2 *__restrict__ partials)
-{
-    const int rhs = blockIdx.y;
-    if (active[rhs] != 1)
-        return;
-    const int e = blockIdx.x * kBlock + threadIdx.x;
-    double acc = 0.0;
-    if (e < nel) {
-        const int par = e / stride, i = e - par * stride;
-        const size_t off = rhs * rhs_bytes + (size_t)par * spinor_chunks(SF) * 16 * stride;
-        const size_t so = ((size_t)rhs * 2 + par) * stride;
-        const float alpha = (float)st[rhs].alpha;
-        cpx<float> vr[4][3], vq[4][3];
-        Spinor<SF>::template load<float, kStreamNT>(r + off, rs ? rs + so : nullptr, stride,
-                                               (unsigned)i, vr);
-        Spinor<SF>::template load<float, kStreamNT>(q + off, qs ? qs + so : nullptr, stride,
-                                               (unsigned)i, vq);
-#pragma unroll
-        for (int s = 0; s < 4; ++s)
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-                vr[s][c] = {fmaf(-alpha, vq[s][c].r, vr[s][c].r),
-                            fmaf(-alpha, vq[s][c].i, vr[s][c].i)};
-        // |r_s|^2 of the stored (rounded) residual
-        if constexpr (SF == WD_F16) {
-            unsigned w[12];
-            const float sc = Spinor<WD_F16>::template store_packed<float, kStreamNT>(
-                r + off, rs + so, stride, (unsigned)i, vr, w);
-            acc = norm2_packed(w, sc);
-        } else {
-            Spinor<SF>::template store<float, kStreamNT>(r + off, nullptr, stride, (unsigned)i, vr);
-#pragma unroll
-            for (int s = 0; s < 4; ++s)
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    acc = fma((double)vr[s][c].r, (double)vr[s][c].r, acc);
-                    acc = fma((double)vr[s][c].i, (double)vr[s][c].i, acc);
-                }
-        }
-    }
-    block_reduce_store(acc, 0.0, partials + (size_t)rhs * gridDim.x + blockIdx.x);
 }
 
 // x (F64) of site i, parity par += Omega^dag v (v: x_s in the sloppy frame,
@@ -361,7 +313,8 @@ __global__ __launch_bounds__(1024) void k_mx_init(const double2 *__restrict__ pa
 // every system's pending set was flushed at the end of the last one)
 __global__ __launch_bounds__(1024) void k_mx_alpha(const double2 *__restrict__ partials, int count,
                                                    MxState *__restrict__ st,
-                                                   int *__restrict__ active, int pos)
+                                                   int *__restrict__ active, int pos,
+                                                   float *__restrict__ alpha_f)
 {
     const int r = blockIdx.x;
     if (pos == 0 && threadIdx.x == 0)
@@ -377,6 +330,7 @@ __global__ __launch_bounds__(1024) void k_mx_alpha(const double2 *__restrict__ p
             return;
         }
         m.alpha = m.rho / pq.x;
+        alpha_f[r] = (float)m.alpha; // the operator's r -= alpha q
         m.ahist[pos] = (float)m.alpha;
         m.pmask |= 1u << pos;
         m.pbuf = pos;
@@ -1257,9 +1211,13 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
     if ((rc = partials.alloc(ctx, sizeof(double2) * (size_t)n * nsb_max)) ||
         (rc = state.alloc(ctx, sizeof(MxState) * n)) ||
         (rc = flags.alloc(ctx, sizeof(int) * (3 * (size_t)n + 4))) ||
-        (rc = diag_d.alloc(ctx, sizeof(double) * n)) || (rc = diag_f.alloc(ctx, sizeof(float) * n)) ||
+        (rc = diag_d.alloc(ctx, sizeof(double) * n)) ||
+        (rc = diag_f.alloc(ctx, sizeof(float) * 2 * n)) ||
         (rc = coef.alloc(ctx, sizeof(double) * 3 * n)))
         return rc;
+    // diag_f: [n] the sloppy operator's diagonal, then [n] alpha of the current
+    // iteration (the operator's r -= alpha q)
+    float *alpha_f = (float *)diag_f.p + n;
     MxState *st = (MxState *)state.p;
     int *active = (int *)flags.p, *reliable = active + n, *check = reliable + n, *stop = check + n;
     double2 *part = (double2 *)partials.p;
@@ -1408,6 +1366,30 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
         m.in_scale = out->scale;
         return wd_launch_hop_sloppy(ctx, m, sp, Us->prec, Us->recon, R, dag, 0, mode);
     };
+    // r_s -= alpha A^dag qq with the partials of |r_s|^2 (HOP_M_AXPY); even-odd:
+    // the intermediate hop goes to the odd half of q
+    auto sloppy_axpy = [&]() -> int {
+        SloppyArgs m = A;
+        m.out = (char *)rs.f->data;
+        m.out_scale = rs.f->scale;
+        m.dotv = (const char *)rs.f->data;
+        m.dotv_scale = rs.f->scale;
+        m.axpy = alpha_f;
+        m.in = m.same = (const char *)qq.f->data;
+        m.in_scale = m.same_scale = qq.f->scale;
+        if (!eo)
+            return wd_launch_hop_sloppy(ctx, m, sp, Us->prec, Us->recon, R, 1, -1, HOP_M_AXPY);
+        SloppyArgs h = m;
+        h.hc = 0.5f;
+        h.out = (char *)q.f->data;
+        h.out_scale = q.f->scale;
+        int e = wd_launch_hop_sloppy(ctx, h, sp, Us->prec, Us->recon, R, 1, 1, HOP_ONLY);
+        if (e)
+            return e;
+        m.in = (const char *)q.f->data;
+        m.in_scale = q.f->scale;
+        return wd_launch_hop_sloppy(ctx, m, sp, Us->prec, Us->recon, R, 1, 0, HOP_M_AXPY);
+    };
 
     // search-direction buffer j: n systems at pbytes * j (scales at pscale * j)
     const size_t pbytes = (size_t)n * srb, pscale = (size_t)n * 2 * stride;
@@ -1435,25 +1417,17 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
         if ((e = wd_check_launch(ctx, "k_mx_xpay")))
             return e;
         // (p, q) = (p, A^dag A p) = |A p|^2, summed over the stored A p by the
-        // launch that writes it, so the A^dag launch reads no third field
+        // launch that writes it; alpha; then r_s -= alpha A^dag (A p) and |r_s|^2
+        // in the A^dag launch
         const wd_field pp = pbuf(pos);
-        if ((e = sloppy_op(0, qq.f, &pp, HOP_M_NORM, nullptr)) ||
-            (e = sloppy_op(1, q.f, qq.f, HOP_M, nullptr)))
+        if ((e = sloppy_op(0, qq.f, &pp, HOP_M_NORM, nullptr)))
             return e;
         hipLaunchKernelGGL(k_mx_alpha, dim3(n), dim3(1024), 0, ctx->stream,
-                           (const double2 *)part, nsb, st, active, pos);
-        if (sp == WD_F16)
-            hipLaunchKernelGGL(k_mx_update<WD_F16>, dim3(eg, n), dim3(kBlock), 0, ctx->stream,
-                               (char *)rs.f->data, rs.f->scale, (const char *)q.f->data,
-                               (const float *)q.f->scale, srb, stride, nel, (const MxState *)st,
-                               (const int *)active, part);
-        else
-            hipLaunchKernelGGL(k_mx_update<WD_F32>, dim3(eg, n), dim3(kBlock), 0, ctx->stream,
-                               (char *)rs.f->data, rs.f->scale, (const char *)q.f->data,
-                               (const float *)q.f->scale, srb, stride, nel, (const MxState *)st,
-                               (const int *)active, part);
+                           (const double2 *)part, nsb, st, active, pos, alpha_f);
+        if ((e = wd_check_launch(ctx, "k_mx_alpha")) || (e = sloppy_axpy()))
+            return e;
         hipLaunchKernelGGL(k_mx_decide, dim3(n), dim3(1024), 0, ctx->stream,
-                           (const double2 *)part, eg, st, active, reliable, prm->max_iter,
+                           (const double2 *)part, nsb, st, active, reliable, prm->max_iter,
                            prm->delta);
         return wd_check_launch(ctx, "mixed cg iteration");
     };

@@ -31,6 +31,11 @@
 //             from the launch that writes t; no dagger form)
 //   HOP_M_NORM      HOP_M plus the partials (|out|^2, 0) (the CG's |A p|^2 from
 //             the launch that writes A p; no dagger form)
+//   HOP_M_AXPY      out = dotv - axpy_r (M in) with the partials (|out|^2, 0)
+//             of the stored out (the CG's r -= alpha A^dag A p in the launch
+//             that computes A^dag A p, which is never stored; out may be the
+//             field dotv, each site is read and written by its own thread;
+//             dagger form only)
 #include "wd_fields.h"
 #include "wd_internal.h"
 #include "wd_sloppy.h"
@@ -262,10 +267,35 @@ __global__ __launch_bounds__(kBlock) void k_hop_sloppy(SloppyArgs a, Geom g, int
                                             fmaf(ar, v[s][c].i, acc[r][s][c].i)};
                 }
             }
+            // HOP_M_AXPY: out = dotv - axpy_r (M in), the elementwise update
+            // fused into the launch that computes M in; F16: with M in as it
+            // would be stored (the arithmetic of an update that reads it back)
+            if constexpr (MODE == HOP_M_AXPY) {
+                const float cr = a.axpy[rhs[r]];
+                if constexpr (SF == WD_F16) {
+                    unsigned q[12], v[12];
+                    const float sq = pack16(acc[r], q);
+                    load16(a.dotv + off, stride, ui, v);
+                    const float sv = load_scale(a.dotv_scale + soff, ui);
+#pragma unroll
+                    for (int k = 0; k < 12; ++k)
+                        acc[r][k / 3][k % 3] = {fmaf(-cr, re16(q[k]) * sq, re16(v[k]) * sv),
+                                                fmaf(-cr, im16(q[k]) * sq, im16(v[k]) * sv)};
+                } else {
+                    cpx<float> v[4][3];
+                    load32(a.dotv + off, stride, ui, v);
+#pragma unroll
+                    for (int s = 0; s < 4; ++s)
+#pragma unroll
+                        for (int c = 0; c < 3; ++c)
+                            acc[r][s][c] = {fmaf(-cr, acc[r][s][c].r, v[s][c].r),
+                                            fmaf(-cr, acc[r][s][c].i, v[s][c].i)};
+                }
+            }
             if constexpr (SF == WD_F16) {
                 unsigned q[12];
                 const float sq = store16(a.out + off, a.out_scale + soff, stride, ui, acc[r], q);
-                if constexpr (MODE == HOP_M_DOT_NORM || MODE == HOP_M_NORM) {
+                if constexpr (MODE == HOP_M_DOT_NORM || MODE == HOP_M_NORM || MODE == HOP_M_AXPY) {
                     // |q|^2 of the stored values: f16 squares are exact in FP32
                     float nq = 0.f;
 #pragma unroll
@@ -295,7 +325,7 @@ __global__ __launch_bounds__(kBlock) void k_hop_sloppy(SloppyArgs a, Geom g, int
                 }
             } else {
                 store32(a.out + off, stride, ui, acc[r]);
-                if constexpr (MODE == HOP_M_DOT_NORM || MODE == HOP_M_NORM) {
+                if constexpr (MODE == HOP_M_DOT_NORM || MODE == HOP_M_NORM || MODE == HOP_M_AXPY) {
 #pragma unroll
                     for (int s = 0; s < 4; ++s)
 #pragma unroll
@@ -326,7 +356,7 @@ __global__ __launch_bounds__(kBlock) void k_hop_sloppy(SloppyArgs a, Geom g, int
             block_sum_store(pr, pi, a.partials + (size_t)rhs[r] * a.nsb + nat);
         if constexpr (MODE == HOP_M_DOT_NORM)
             block_sum_store(nn, 0.0, a.partials2 + (size_t)rhs[r] * a.nsb + nat);
-        if constexpr (MODE == HOP_M_NORM)
+        if constexpr (MODE == HOP_M_NORM || MODE == HOP_M_AXPY)
             block_sum_store(nn, 0.0, a.partials + (size_t)rhs[r] * a.nsb + nat);
     }
 }
@@ -355,6 +385,13 @@ void launch_r(wd_ctx *ctx, const SloppyArgs &a, const Geom &g, int cb_first, int
         // |A p|^2 of the CG's A p: no dagger form is needed
         if constexpr (!DAG)
             hipLaunchKernelGGL((k_hop_sloppy<SF, GF, R, false, HOP_M_NORM>), dim3(grid),
+                               dim3(kBlock), 0, ctx->stream, a, g, cb_first, nparity,
+                               ctx->xcd_remap);
+        break;
+    case HOP_M_AXPY:
+        // the CG's r -= alpha A^dag (A p): the dagger form only
+        if constexpr (DAG)
+            hipLaunchKernelGGL((k_hop_sloppy<SF, GF, R, true, HOP_M_AXPY>), dim3(grid),
                                dim3(kBlock), 0, ctx->stream, a, g, cb_first, nparity,
                                ctx->xcd_remap);
         break;
@@ -440,6 +477,9 @@ int wd_launch_hop_sloppy(wd_ctx *ctx, SloppyArgs a, int spinor_prec, int gauge_p
                  "sloppy hop: the norm mode has no dagger form and needs partials2");
     WD_ARG_CHECK(ctx, mode != HOP_M_NORM || (!dagger && a.partials),
                  "sloppy hop: the norm-only mode has no dagger form and needs partials");
+    WD_ARG_CHECK(ctx, mode != HOP_M_AXPY || (dagger && a.partials && a.dotv && a.axpy),
+                 "sloppy hop: the axpy mode is the dagger form only and needs partials, dotv "
+                 "and axpy");
     a.one = 1.0f;
     a.tg = recon == WD_RECON_12T;
     WD_ARG_CHECK(ctx, wd_sloppy_supported(spinor_prec, gauge_prec, recon),

@@ -32,7 +32,8 @@
 //
 // Modes as for the other sloppy kernel: HOP_ONLY, HOP_M, HOP_M_DOT (per-system
 // FP64 partials of sum conj(dotv) out over the stored values), HOP_M_DOT_NORM
-// (plus |out|^2; no dagger form), HOP_M_NORM (|out|^2 only; no dagger form).
+// (plus |out|^2; no dagger form), HOP_M_NORM (|out|^2 only; no dagger form),
+// HOP_M_AXPY (out = dotv - axpy_r (M in) and its |out|^2; dagger form only).
 // The partials are per wavefront, kLoopWaves per workgroup ([rhs][nsb * kLoopWaves]), summed by shuffles without a
 // workgroup barrier per system.
 #include "wd_fields.h"
@@ -458,9 +459,9 @@ __device__ __forceinline__ void loop_block(const SloppyArgs &a, const Geom &g, i
     // stores leave the memory counter in the order they were issued, so a
     // wait for a load issued after the stores would also wait for the
     // stores to complete.
-    // (two directions ahead where the registers allow; the |out|^2 launch
-    // would spill with two)
-    constexpr int PRE = (K == 1 && MODE != HOP_M_NORM) ? 2 : K;
+    // (two directions ahead where the registers allow; the |out|^2 and axpy
+    // launches would spill with two)
+    constexpr int PRE = (K == 1 && MODE != HOP_M_NORM && MODE != HOP_M_AXPY) ? 2 : K;
     Nbr nb[K + 1];
     int r = next_applied(r0);
     if (site_ok && r < r1) {
@@ -509,24 +510,43 @@ __device__ __forceinline__ void loop_block(const SloppyArgs &a, const Geom &g, i
                 }
             }
             unsigned q[12];
-            const float sq = pack16(acc, q);
-            // the dot product's operand, then the next system's first
-            // neighbours, are loaded before the stores
+            float sq = pack16(acc, q);
+            // the dot product's operand (HOP_M_AXPY: the updated field), then
+            // the next system's first neighbours, are loaded before the stores
             unsigned dv[12];
             float sd = 0.f;
-            if constexpr (MODE == HOP_M_DOT || MODE == HOP_M_DOT_NORM) {
+            if constexpr (MODE == HOP_M_DOT || MODE == HOP_M_DOT_NORM || MODE == HOP_M_AXPY) {
+                // (not hoisted above the packing: the axpy launch would spill)
+                if constexpr (MODE == HOP_M_AXPY)
+                    __builtin_amdgcn_sched_barrier(0);
                 load16(a.dotv + off, stride, ui, dv);
                 sd = load_scale(a.dotv_scale + soff, ui);
             }
-            if (rn < r1) {
-                const char *inb_n = in_p + (size_t)rn * a.rhs_bytes;
-                const float *isc_n = isc_p + (size_t)rn * 2 * stride;
+            auto prefetch_next = [&]() {
+                if (rn < r1) {
+                    const char *inb_n = in_p + (size_t)rn * a.rhs_bytes;
+                    const float *isc_n = isc_p + (size_t)rn * 2 * stride;
 #pragma unroll
-                for (int d = 0; d < PRE; ++d)
-                    load_nbr(inb_n, isc_n, j[d], stride, nb[d]);
+                    for (int d = 0; d < PRE; ++d)
+                        load_nbr(inb_n, isc_n, j[d], stride, nb[d]);
+                }
+            };
+            if constexpr (MODE == HOP_M_AXPY) {
+                // out = dotv - axpy_r (M in), M in as rounded for storage: the
+                // arithmetic of an elementwise update reading M in back; the
+                // next system's loads follow (before them, dotv would share
+                // the registers with them and spill)
+                const float cr = a.axpy[r];
+                cpx<float> t[4][3];
+#pragma unroll
+                for (int k = 0; k < 12; ++k)
+                    t[k / 3][k % 3] = {fmaf(-cr, re16(q[k]) * sq, re16(dv[k]) * sd),
+                                       fmaf(-cr, im16(q[k]) * sq, im16(dv[k]) * sd)};
+                sq = pack16(t, q);
             }
+            prefetch_next();
             store16w(a.out + off, a.out_scale + soff, stride, ui, q, sq);
-            if constexpr (MODE == HOP_M_DOT_NORM || MODE == HOP_M_NORM) {
+            if constexpr (MODE == HOP_M_DOT_NORM || MODE == HOP_M_NORM || MODE == HOP_M_AXPY) {
                 float nq = 0.f;
 #pragma unroll
                 for (int k = 0; k < 12; ++k) {
@@ -556,7 +576,7 @@ __device__ __forceinline__ void loop_block(const SloppyArgs &a, const Geom &g, i
             wave_sum_store(pr, pi, a.partials + pidx);
         if constexpr (MODE == HOP_M_DOT_NORM)
             wave_sum_store(nn, 0.0, a.partials2 + pidx);
-        if constexpr (MODE == HOP_M_NORM)
+        if constexpr (MODE == HOP_M_NORM || MODE == HOP_M_AXPY)
             wave_sum_store(nn, 0.0, a.partials + pidx);
         r = rn;
     }
@@ -634,6 +654,12 @@ void launch_mode(wd_ctx *ctx, const SloppyArgs &a, const Geom &g, int cb_first, 
                                dim3(kBlock), 0, ctx->stream, a, g, cb_first, nparity,
                                ctx->xcd_remap, per, ctx->parity_il);
         break;
+    case HOP_M_AXPY:
+        if constexpr (DAG)
+            hipLaunchKernelGGL((k_hop_sloppy_loop<true, HOP_M_AXPY, K, LF>), dim3(grid),
+                               dim3(kBlock), 0, ctx->stream, a, g, cb_first, nparity,
+                               ctx->xcd_remap, per, ctx->parity_il);
+        break;
     default:
         hipLaunchKernelGGL((k_hop_sloppy_loop<DAG, HOP_M_DOT, K, LF>), dim3(grid), dim3(kBlock), 0,
                            ctx->stream, a, g, cb_first, nparity, ctx->xcd_remap, per, ctx->parity_il);
@@ -677,8 +703,11 @@ int wd_launch_hop_sloppy_loop(wd_ctx *ctx, const SloppyArgs &a, const Geom &g, i
     }
     // automatic prefetch: 1 ahead at 3 wavefronts per SIMD for the hop and M
     // launches; 3 ahead at 2 for the dot launches, which spill 20-23
-    // registers at 3 wavefronts per SIMD
-    const bool dot = mode == HOP_M_DOT || mode == HOP_M_DOT_NORM; // HOP_M_NORM fits at 3
+    // registers at 3 wavefronts per SIMD, and for the axpy launch (which fits
+    // at 3 with one direction ahead; measured on MI355X, 48 systems at
+    // 24^3x48, Schur CG 0.508 s either way, 3 ahead within noise)
+    const bool dot = mode == HOP_M_DOT || mode == HOP_M_DOT_NORM ||
+                     mode == HOP_M_AXPY; // HOP_M_NORM fits at 3
 
     const int K = ctx->loop_prefetch ? ctx->loop_prefetch : dot ? 3 : 1;
     if (K == 3) {

@@ -235,4 +235,5 @@ enum HopMode {
     HOP_M_DOT = 2,    // as HOP_M, plus per-block partial of sum conj(dotv) * out
     HOP_M_DOT_NORM = 3, // as HOP_M_DOT, plus per-block partial of |out|^2 (sloppy kernel)
     HOP_M_NORM = 4,   // as HOP_M, plus per-block partial (|out|^2, 0) (sloppy kernels, no dagger)
+    HOP_M_AXPY = 5,   // out = dotv - axpy_r (M in), partial (|out|^2, 0) (sloppy kernels, dagger only)
 };

@@ -11,8 +11,9 @@ struct SloppyArgs {
     const float *in_scale;
     const char *same;        // HOP_M / HOP_M_DOT: the diagonal term's input
     const float *same_scale;
-    const char *dotv;        // HOP_M_DOT: partial of sum conj(dotv) * out
-    const float *dotv_scale;
+    const char *dotv;        // HOP_M_DOT: partial of sum conj(dotv) * out; HOP_M_AXPY: the
+    const float *dotv_scale; //   field the update is applied to (out = dotv - axpy_r M in)
+    const float *axpy;       // [nrhs] HOP_M_AXPY coefficients
     size_t rhs_bytes;        // spinor bytes per rhs (all fields share the format)
     const char *U;           // gauge field
     const float *diag;       // [nrhs] a_r = 4 + m_r
