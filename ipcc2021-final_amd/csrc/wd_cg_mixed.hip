@@ -1188,13 +1188,14 @@ extern "C" int wd_cg_mixed(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd
     // iterations per group of pending x_s steps: one search-direction buffer each
     const int npend = std::min(slot_every, kPend);
     // device work vectors (even-odd: the Schur system lives on the even half;
-    // the odd half of an operator's output buffer holds its intermediate hop);
-    // p holds npend buffers of n systems
+    // the odd half of an operator's output buffer holds its intermediate hop,
+    // q only that of the A^dag launch); p holds npend buffers of n systems
     FieldBuf Mdb, Mfull, t64, q64, rs, p, q, qq, xs;
     if ((rc = wd_field_create_work(ctx, &Mdb.f, WD_F64, n)) || (rc = wd_field_create_work(ctx, &t64.f, WD_F64, n)) ||
         (rc = wd_field_create_work(ctx, &q64.f, WD_F64, n)) || (rc = wd_field_create_work(ctx, &rs.f, sp, n)) ||
         (rc = wd_field_create_work(ctx, &p.f, sp, npend * n)) ||
-        (rc = wd_field_create_work(ctx, &q.f, sp, n)) || (rc = wd_field_create_work(ctx, &qq.f, sp, n)) ||
+        (eo && (rc = wd_field_create_work(ctx, &q.f, sp, n))) ||
+        (rc = wd_field_create_work(ctx, &qq.f, sp, n)) ||
         (rc = wd_field_create_work(ctx, &xs.f, WD_F32, n)) ||
         (eo && (rc = wd_field_create_work(ctx, &Mfull.f, WD_F64, n))))
         return rc;
