@@ -162,6 +162,34 @@ __global__ __launch_bounds__(kBlock) void k_mx_xpay(char *pb, float *ps, size_t 
                                                  (unsigned)i, vp);
 }
 
+// The packed words and scale of an F16 site (converted where they are used,
+// so a group's pending directions can be in flight together in few registers)
+struct Raw16 {
+    unsigned w[12];
+    float s;
+};
+__device__ __forceinline__ void load_raw16(const char *base, const float *scale, int stride,
+                                           unsigned i, Raw16 &v)
+{
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const u4v c = ldg<kStreamNT>(
+            reinterpret_cast<const u4v *>(base + ((size_t)j * (unsigned)stride + i) * 16));
+        v.w[4 * j] = c.x;
+        v.w[4 * j + 1] = c.y;
+        v.w[4 * j + 2] = c.z;
+        v.w[4 * j + 3] = c.w;
+    }
+    v.s = ldg<kStreamNT>(scale + i);
+}
+// the values Spinor<WD_F16>::load returns for the same site
+__device__ __forceinline__ void unpack_raw16(const Raw16 &v, cpx<float> (&out)[4][3])
+{
+#pragma unroll
+    for (int k = 0; k < 12; ++k)
+        out[k / 3][k % 3] = h2f(v.w[k], v.s);
+}
+
 // x (F64) of site i, parity par += Omega^dag v (v: x_s in the sloppy frame,
 // omega == nullptr: the identity); x_s = 0
 __device__ __forceinline__ void fold_into_x(double2 *__restrict__ xp, char *__restrict__ sp,
@@ -213,19 +241,42 @@ __global__ __launch_bounds__(kBlock) void k_mx_flush(double2 *__restrict__ x,
     const size_t so = ((size_t)rhs * 2 + par) * stride;
     cpx<float> vs[4][3];
     Spinor<WD_F32>::template load<float, kStreamNT>(sp, nullptr, stride, (unsigned)i, vs);
-    for (int j = 0; j < kPend; ++j) {
-        if (!((pm >> j) & 1u))
-            continue;
+    auto step = [&](int j, const cpx<float> (&vp)[4][3]) {
         const float alpha = st[rhs].ahist[j];
-        cpx<float> vp[4][3];
-        Spinor<SF>::template load<float, kStreamNT>(pb + j * pbytes + off,
-                                                    ps ? ps + j * pscale + so : nullptr, stride,
-                                                    (unsigned)i, vp);
 #pragma unroll
         for (int s = 0; s < 4; ++s)
 #pragma unroll
             for (int c = 0; c < 3; ++c)
                 vs[s][c] = {fmaf(alpha, vp[s][c].r, vs[s][c].r), fmaf(alpha, vp[s][c].i, vs[s][c].i)};
+    };
+    if constexpr (SF == WD_F16) {
+        // F16: four pending directions at a time, their loads issued together
+        // (one at a time, each load waited for the previous step)
+#pragma unroll
+        for (int j0 = 0; j0 < kPend; j0 += 4) {
+            Raw16 raw[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if ((pm >> (j0 + u)) & 1u)
+                    load_raw16(pb + (j0 + u) * pbytes + off, ps + (j0 + u) * pscale + so, stride,
+                               (unsigned)i, raw[u]);
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if ((pm >> (j0 + u)) & 1u) {
+                    cpx<float> vp[4][3];
+                    unpack_raw16(raw[u], vp);
+                    step(j0 + u, vp);
+                }
+        }
+    } else {
+        for (int j = 0; j < kPend; ++j) {
+            if (!((pm >> j) & 1u))
+                continue;
+            cpx<float> vp[4][3];
+            Spinor<SF>::template load<float, kStreamNT>(pb + j * pbytes + off, nullptr, stride,
+                                                        (unsigned)i, vp);
+            step(j, vp);
+        }
     }
     if (due)
         fold_into_x(x + rhs * x_rhs_elems + (size_t)par * 12 * stride, sp, omega, par, i, stride,
@@ -612,9 +663,11 @@ __device__ __forceinline__ void bs_xstep(cpx<float> (&vx)[4][3], const cpx<float
     for (int s = 0; s < 4; ++s)
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
+            // explicit FMAs: the same rounding wherever the step is inlined
             const cpx<float> P = vp[s][c], S = vs[s][c];
-            vx[s][c] = {vx[s][c].r + (ar * P.r - ai * P.i) + (wr * S.r - wi * S.i),
-                        vx[s][c].i + (ar * P.i + ai * P.r) + (wr * S.i + wi * S.r)};
+            const float pr = fmaf(ar, P.r, -(ai * P.i)), pi = fmaf(ar, P.i, ai * P.r);
+            const float sr = fmaf(wr, S.r, -(wi * S.i)), si = fmaf(wr, S.i, wi * S.r);
+            vx[s][c] = {(vx[s][c].r + pr) + sr, (vx[s][c].i + pi) + si};
         }
 }
 
@@ -700,19 +753,44 @@ __global__ __launch_bounds__(kBlock) void k_bs_flush(
     char *sp = xs + rhs * xs_rhs_bytes + (size_t)par * spinor_chunks(WD_F32) * 16 * stride;
     cpx<float> vx[4][3];
     Spinor<WD_F32>::template load<float, kStreamNT>(sp, nullptr, stride, (unsigned)i, vx);
-    for (int j = 0; j < kPend; ++j) {
-        if (!((pm >> j) & 1u))
-            continue;
-        const float ar = st[rhs].ahist[j][0], ai = st[rhs].ahist[j][1];
-        const float wr = st[rhs].whist[j][0], wi = st[rhs].whist[j][1];
-        cpx<float> vp[4][3], vs[4][3];
-        Spinor<SF>::template load<float, kStreamNT>(pb + j * bytes + off,
-                                                    ps ? ps + j * scale + so : nullptr, stride,
-                                                    (unsigned)i, vp);
-        Spinor<SF>::template load<float, kStreamNT>(sb + j * bytes + off,
-                                                    ss ? ss + j * scale + so : nullptr, stride,
-                                                    (unsigned)i, vs);
-        bs_xstep(vx, vp, vs, ar, ai, wr, wi);
+    auto step = [&](int j, const cpx<float> (&vp)[4][3], const cpx<float> (&vs)[4][3]) {
+        bs_xstep(vx, vp, vs, st[rhs].ahist[j][0], st[rhs].ahist[j][1], st[rhs].whist[j][0],
+                 st[rhs].whist[j][1]);
+    };
+    if constexpr (SF == WD_F16) {
+        // F16: two pending steps (p and s each) at a time, their loads issued
+        // together
+#pragma unroll
+        for (int j0 = 0; j0 < kPend; j0 += 2) {
+            Raw16 rp[2], rs[2];
+#pragma unroll
+            for (int u = 0; u < 2; ++u)
+                if ((pm >> (j0 + u)) & 1u) {
+                    load_raw16(pb + (j0 + u) * bytes + off, ps + (j0 + u) * scale + so, stride,
+                               (unsigned)i, rp[u]);
+                    load_raw16(sb + (j0 + u) * bytes + off, ss + (j0 + u) * scale + so, stride,
+                               (unsigned)i, rs[u]);
+                }
+#pragma unroll
+            for (int u = 0; u < 2; ++u)
+                if ((pm >> (j0 + u)) & 1u) {
+                    cpx<float> vp[4][3], vs[4][3];
+                    unpack_raw16(rp[u], vp);
+                    unpack_raw16(rs[u], vs);
+                    step(j0 + u, vp, vs);
+                }
+        }
+    } else {
+        for (int j = 0; j < kPend; ++j) {
+            if (!((pm >> j) & 1u))
+                continue;
+            cpx<float> vp[4][3], vs[4][3];
+            Spinor<SF>::template load<float, kStreamNT>(pb + j * bytes + off, nullptr, stride,
+                                                        (unsigned)i, vp);
+            Spinor<SF>::template load<float, kStreamNT>(sb + j * bytes + off, nullptr, stride,
+                                                        (unsigned)i, vs);
+            step(j, vp, vs);
+        }
     }
     if (due)
         fold_into_x(x + rhs * x_rhs_elems + (size_t)par * 12 * stride, sp, omega, par, i, stride,
