@@ -1,16 +1,17 @@
 // wd_cg.hip -- FP64 CGNR on M^dagger M, the reference's CGinvert
 // (src/invert.cpp:25-121) as a device-resident loop.
 //
-// One iteration is six launches, all reading their scalars from device memory
+// One iteration is five launches, all reading their scalars from device memory
 // so the host never waits inside the loop:
 //   1 k_cg_xpay      x += alpha p (the previous iteration's update, deferred to
 //                    here where p is read anyway), then p = r + beta p
 //                    (alpha = beta = 0 and p = 0 on the first iteration)
-//   2 hop  M         qq = M p                  (fused (4+m) diagonal + hop)
-//   3 hop  M^dag+dot q = M^dag qq, partial (p, q) per workgroup
-//   4 k_cg_alpha     alpha = rho / (p, q)      (complex, as the reference)
-//   5 k_cg_update    r -= alpha q, partial |r|^2
-//   6 k_cg_beta      |r|, history, stop test |r| < acc, beta = |r|^2 / rho, rho = |r|^2
+//   2 hop  M+norm    qq = M p, partial |qq|^2 per workgroup (HOP_M_NORM)
+//   3 k_cg_alpha     alpha = rho / |M p|^2     ((p, M^dag M p) of the reference,
+//                    src/invert.cpp:72-77, in the form whose value is real)
+//   4 hop  M^dag+axpy r -= alpha M^dag qq, partial |r|^2 (HOP_M_AXPY; the
+//                    reference's q = M^dag M p is never stored)
+//   5 k_cg_beta      |r|, history, stop test |r| < acc, beta = |r|^2 / rho, rho = |r|^2
 // rho = (r, r) is carried from step 6 to the next iteration's step 4; the
 // reference recomputes the same sum with vector_p (src/invert.cpp:68, :82).
 // Deferring x += alpha p changes no value (same alpha, same p) and saves one
@@ -105,7 +106,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_xfinal(double2 *__restrict__ x,
         stv(x, i, caxpy(ldv(x, i), ar, ai, ldv(p, i)));
 }
 
-// alpha = rho / (p, q); breakdown when (p, q) is not finite or its real part <= 0
+// alpha = rho / (p, M^dag M p) = rho / |M p|^2 (partials of |M p|^2);
+// breakdown when |M p|^2 is not finite or not positive
 __global__ __launch_bounds__(1024) void k_cg_alpha(const double2 *__restrict__ partials, int count,
                                                    CGState *__restrict__ st)
 {
@@ -113,47 +115,13 @@ __global__ __launch_bounds__(1024) void k_cg_alpha(const double2 *__restrict__ p
         return;
     const double2 pq = sum_partials_1024(partials, count);
     if (threadIdx.x == 0) {
-        if (!isfinite(pq.x) || !isfinite(pq.y) || !(pq.x > 0.0)) {
+        if (!isfinite(pq.x) || !(pq.x > 0.0)) {
             st->alpha = make_double2(0.0, 0.0); // no x update for this iteration
             st->stop = 2;
             return;
         }
-        const double den = pq.x * pq.x + pq.y * pq.y;
-        // (rho + 0i) / pq
-        st->alpha = make_double2(st->rho * pq.x / den, -st->rho * pq.y / den);
+        st->alpha = make_double2(st->rho / pq.x, 0.0);
     }
-}
-
-// r -= alpha q ; partial |r|^2 (two elements per trip, loads first)
-__global__ __launch_bounds__(kBlock) void k_cg_update(double2 *__restrict__ r,
-                                                      const double2 *__restrict__ q,
-                                                      const CGState *__restrict__ st, size_t n,
-                                                      double2 *__restrict__ partials)
-{
-    if (st->stop)
-        return;
-    const double ar = -st->alpha.x, ai = -st->alpha.y;
-    double s = 0.0;
-    const size_t step = (size_t)gridDim.x * kBlock;
-    size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x;
-    for (; i + step < n; i += 2 * step) {
-        const d2v q0 = ldv(q, i), q1 = ldv(q, i + step);
-        const d2v r0 = ldv(r, i), r1 = ldv(r, i + step);
-        const d2v o0 = caxpy(r0, ar, ai, q0), o1 = caxpy(r1, ar, ai, q1);
-        stv(r, i, o0);
-        stv(r, i + step, o1);
-        s = fma(o0.x, o0.x, s);
-        s = fma(o0.y, o0.y, s);
-        s = fma(o1.x, o1.x, s);
-        s = fma(o1.y, o1.y, s);
-    }
-    if (i < n) {
-        const d2v o0 = caxpy(ldv(r, i), ar, ai, ldv(q, i));
-        stv(r, i, o0);
-        s = fma(o0.x, o0.x, s);
-        s = fma(o0.y, o0.y, s);
-    }
-    block_reduce_store(s, 0.0, partials + blockIdx.x);
 }
 
 __global__ __launch_bounds__(1024) void k_cg_beta(const double2 *__restrict__ partials, int count,
@@ -381,13 +349,11 @@ extern "C" int wd_cg(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd_gauge
             return rc;
     }
     const int hop_blocks = wd_hop_grid(ctx, 2);
-    // streaming kernels (2 elements per thread and trip): one trip per thread
-    // for xpay; the reduction grid of update is capped by the partials buffer.
-    // Measured on MI355X: ~81 % of HBM peak from about 16k workgroups up,
-    // ~68 % at 1k.
+    // the streaming kernel (2 elements per thread and trip): one trip per
+    // thread for xpay.  Measured on MI355X: ~81 % of HBM peak from about 16k
+    // workgroups up, ~68 % at 1k.
     const size_t per_blk = (size_t)kBlock * 2;
     const int sg = (int)std::max<size_t>(1, std::min<size_t>((n + per_blk - 1) / per_blk, 65536));
-    const int ug = std::min(sg, std::min(ctx->max_partials, kStreamPartials));
     const int *stop = &st->stop;
 
     auto launch_iteration = [&]() -> int {
@@ -396,22 +362,17 @@ extern "C" int wd_cg(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd_gauge
                            n);
         if ((e = wd_check_launch(ctx, "k_cg_xpay")))
             return e;
-        if ((e = wd_launch_hop_f64(ctx, qq.p, p.p, Ud, 0, -1, HOP_M, a, p.p, nullptr, nullptr,
-                                   stop)))
-            return e;
-        if ((e = wd_launch_hop_f64(ctx, q.p, qq.p, Ud, 1, -1, HOP_M_DOT, a, qq.p, p.p, partials,
-                                   stop)))
+        if ((e = wd_launch_hop_f64(ctx, qq.p, p.p, Ud, 0, -1, HOP_M_NORM, a, p.p, nullptr,
+                                   partials, stop)))
             return e;
         hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(1024), 0, ctx->stream, partials, hop_blocks,
                            st);
-        if ((e = wd_check_launch(ctx, "k_cg_alpha")))
+        if ((e = wd_check_launch(ctx, "k_cg_alpha")) ||
+            (e = wd_launch_hop_f64_axpy(ctx, r.p, qq.p, Ud, 1, -1, a, qq.p, &st->alpha.x,
+                                        partials, stop)))
             return e;
-        hipLaunchKernelGGL(k_cg_update, dim3(ug), dim3(kBlock), 0, ctx->stream, r.p, q.p, st, n,
-                           partials);
-        if ((e = wd_check_launch(ctx, "k_cg_update")))
-            return e;
-        hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(1024), 0, ctx->stream, partials, ug, st,
-                           d_hist);
+        hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(1024), 0, ctx->stream, partials, hop_blocks,
+                           st, d_hist);
         return wd_check_launch(ctx, "k_cg_beta");
     };
 
@@ -527,7 +488,7 @@ extern "C" int wd_cg_eo(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd_ga
         if (e)
             return e;
         return wd_launch_hop_f64c(ctx, out, t.p, Ud, dag, 0, mode, a2, -0.5, in, dotv,
-                                  mode == HOP_M_DOT ? partials : nullptr, stop);
+                                  mode == HOP_ONLY || mode == HOP_M ? nullptr : partials, stop);
     };
     // |M^dag b - M^dag M x| of the full system
     auto full_residual = [&](double *out) -> int {
@@ -580,7 +541,6 @@ extern "C" int wd_cg_eo(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd_ga
     const int hop_blocks = wd_hop_grid(ctx, 1);
     const size_t per_blk = (size_t)kBlock * 2;
     const int sg = (int)std::max<size_t>(1, std::min<size_t>((ne + per_blk - 1) / per_blk, 65536));
-    const int ug = std::min(sg, std::min(ctx->max_partials, kStreamPartials));
     const int eg = wd_elem_grid(ne), rg = wd_red_grid(ne);
     const int *stop = &st->stop;
     double *hist_at = d_hist;
@@ -591,19 +551,20 @@ extern "C" int wd_cg_eo(wd_ctx *ctx, wd_field *x, const wd_field *b, const wd_ga
                            ne);
         if ((e = wd_check_launch(ctx, "k_cg_xpay")))
             return e;
-        if ((e = apply_A(0, p.p, qq.p, HOP_M, nullptr, stop)) ||
-            (e = apply_A(1, qq.p, q.p, HOP_M_DOT, p.p, stop)))
+        // qq = A p with |A p|^2; alpha; r -= alpha A^dag qq with |r|^2 (the
+        // second A^dag launch updates r instead of storing A^dag A p)
+        if ((e = apply_A(0, p.p, qq.p, HOP_M_NORM, nullptr, stop)))
             return e;
         hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(1024), 0, ctx->stream, partials, hop_blocks,
                            st);
-        if ((e = wd_check_launch(ctx, "k_cg_alpha")))
+        if ((e = wd_check_launch(ctx, "k_cg_alpha")) ||
+            (e = wd_launch_hop_f64c(ctx, t.p, qq.p, Ud, 1, 1, HOP_ONLY, 0.0, 0.5, nullptr, nullptr,
+                                    nullptr, stop)) ||
+            (e = wd_launch_hop_f64_axpy(ctx, r.p, t.p, Ud, 1, 0, a2, qq.p, &st->alpha.x, partials,
+                                        stop)))
             return e;
-        hipLaunchKernelGGL(k_cg_update, dim3(ug), dim3(kBlock), 0, ctx->stream, r.p, q.p, st, ne,
-                           partials);
-        if ((e = wd_check_launch(ctx, "k_cg_update")))
-            return e;
-        hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(1024), 0, ctx->stream, partials, ug, st,
-                           hist_at);
+        hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(1024), 0, ctx->stream, partials, hop_blocks,
+                           st, hist_at);
         return wd_check_launch(ctx, "k_cg_beta");
     };
 

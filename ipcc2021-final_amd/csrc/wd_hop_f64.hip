@@ -13,6 +13,7 @@
 //   HOP_M     out[p] = a * same[p] - 1/2 hop(in)          DslashNew, a = 4 + m
 //   HOP_M_DOT HOP_M plus a per-workgroup partial of sum conj(dotv) * out, the
 //             (p, q) product of the CG step fused into q = M^dagger (M p).
+#include "wd_blas.h"
 #include "wd_dirac.h"
 #include "wd_fields.h"
 #include "wd_internal.h"
@@ -225,15 +226,38 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 1))) 
                 }
         } else {
             const double2 *same_p = same + (long)p * 12 * g.stride;
+            // HOP_M_AXPY: out = dotv - alpha res (res = M in is not stored)
+            const double2 *dv_p = dotv + (long)p * 12 * g.stride;
+            const double al = MODE == HOP_M_AXPY ? *b.axpy : 0.0;
 #pragma unroll
             for (int s = 0; s < 4; ++s)
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     const cpx<double> v = ld(same_p, f64_idx(s * 3 + c, (unsigned)i, 12));
                     res[s][c] = {fma(a, v.r, hc * acc[s][c].r), fma(a, v.i, hc * acc[s][c].i)};
+                    if constexpr (MODE == HOP_M_AXPY) {
+                        const cpx<double> d = ld(dv_p, f64_idx(s * 3 + c, (unsigned)i, 12));
+                        res[s][c] = {fma(-al, res[s][c].r, d.r), fma(-al, res[s][c].i, d.i)};
+                    }
                     st(out_p, f64_idx(s * 3 + c, (unsigned)i, 12), res[s][c]);
                 }
         }
+    }
+
+    if constexpr (MODE == HOP_M_NORM || MODE == HOP_M_AXPY) {
+        // partial (|out|^2, 0) of the stored values for this workgroup
+        double nn = 0.0;
+        if (active) {
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    nn = fma(res[s][c].r, res[s][c].r, nn);
+                    nn = fma(res[s][c].i, res[s][c].i, nn);
+                }
+        }
+        // (partials already point at this system's row)
+        block_reduce_store(nn, 0.0, partials + (nparity == 2 ? p * bpp : 0) + blk);
     }
 
     if constexpr (MODE == HOP_M_DOT) {
@@ -463,7 +487,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 1))) 
     double2 *__restrict__ out, const double2 *__restrict__ in, const double2 *__restrict__ U,
     Geom g, double a, double hc, const double2 *__restrict__ same,
     const double2 *__restrict__ dotv, double2 *__restrict__ partials, int Tb, int ncb, int nwork,
-    int remap, const int *__restrict__ stop)
+    int remap, const int *__restrict__ stop, const double *__restrict__ axpy)
 {
     if (stop && *stop)
         return;
@@ -597,11 +621,32 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 1))) 
                 }
         }
         if (live) {
+            if constexpr (MODE == HOP_M_AXPY) {
+                // out = dotv - alpha res (res = M in is not stored), |out|^2
+                const double2 *dv = dotv + (long)p * 12 * stride;
+                const double al = *axpy;
+#pragma unroll
+                for (int s = 0; s < 4; ++s)
+#pragma unroll
+                    for (int cc = 0; cc < 3; ++cc) {
+                        const cpx<double> d = ld(dv, f64_idx(s * 3 + cc, (unsigned)i, 12));
+                        res[s][cc] = {fma(-al, res[s][cc].r, d.r), fma(-al, res[s][cc].i, d.i)};
+                    }
+            }
 #pragma unroll
             for (int s = 0; s < 4; ++s)
 #pragma unroll
                 for (int cc = 0; cc < 3; ++cc)
                     st(out_p, f64_idx(s * 3 + cc, (unsigned)i, 12), res[s][cc]);
+            if constexpr (MODE == HOP_M_NORM || MODE == HOP_M_AXPY) {
+#pragma unroll
+                for (int s = 0; s < 4; ++s)
+#pragma unroll
+                    for (int cc = 0; cc < 3; ++cc) {
+                        pr = fma(res[s][cc].r, res[s][cc].r, pr);
+                        pr = fma(res[s][cc].i, res[s][cc].i, pr);
+                    }
+            }
             if constexpr (MODE == HOP_M_DOT) {
                 const double2 *dv = dotv + (long)p * 12 * stride;
 #pragma unroll
@@ -623,7 +668,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 1))) 
             su3_mul_half<true>(W, h, rc);
         }
     }
-    if constexpr (MODE == HOP_M_DOT) {
+    if constexpr (MODE == HOP_M_DOT || MODE == HOP_M_NORM || MODE == HOP_M_AXPY) {
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) {
             pr += __shfl_xor(pr, off, 64);
@@ -664,21 +709,26 @@ int launch_mode(wd_ctx *ctx, double2 *out, const double2 *in, const double2 *U, 
     const int bpp = (g.Vh + kBlock - 1) / kBlock;
     const int grid = bpp * nparity;
     const int cb_first = (cb < 0) ? 0 : cb;
-    if (mode == HOP_M_DOT && partials == ctx->partials && (long)grid * nrhs > ctx->max_partials)
+    const bool red = mode == HOP_M_DOT || mode == HOP_M_NORM || mode == HOP_M_AXPY;
+    if (red && partials == ctx->partials && (long)grid * nrhs > ctx->max_partials)
         return wd_set_error(ctx, WD_EARG, "hop: too many workgroups for the partials buffer");
     if (nparity == 2 && nrhs == 1 && ctx->ts_tb > 0 && !b.mask && !b.a_rhs && !b.c_rhs &&
-        g.Lt % ctx->ts_tb == 0 && (mode == HOP_M || mode == HOP_M_DOT)) {
+        g.Lt % ctx->ts_tb == 0 && (mode == HOP_M || red)) {
         const int tb = ctx->ts_tb;
         const int ncb = (g.Lxh * g.Ly * g.Lz + kBlock - 1) / kBlock;
         const int gts = tstream_grid(g, tb);
         const int glaunch = (gts + kNumXCD - 1) / kNumXCD * kNumXCD;
-        if (mode == HOP_M_DOT && partials == ctx->partials && gts > ctx->max_partials)
+        if (red && partials == ctx->partials && gts > ctx->max_partials)
             return wd_set_error(ctx, WD_EARG, "hop: too many workgroups for the partials buffer");
 #define WD_HOP_TS_LAUNCH(M)                                                                        \
     hipLaunchKernelGGL((k_hop_f64_ts<DAG, M, LN>), dim3(glaunch), dim3(kBlock), 0, ctx->stream, out, in, \
-                       U, g, a, hc, same, dotv, partials, tb, ncb, gts, ctx->xcd_remap, stop)
+                       U, g, a, hc, same, dotv, partials, tb, ncb, gts, ctx->xcd_remap, stop, b.axpy)
         if (mode == HOP_M)
             WD_HOP_TS_LAUNCH(HOP_M);
+        else if (mode == HOP_M_NORM)
+            WD_HOP_TS_LAUNCH(HOP_M_NORM);
+        else if (mode == HOP_M_AXPY)
+            WD_HOP_TS_LAUNCH(HOP_M_AXPY);
         else
             WD_HOP_TS_LAUNCH(HOP_M_DOT);
 #undef WD_HOP_TS_LAUNCH
@@ -721,6 +771,18 @@ int launch_mode(wd_ctx *ctx, double2 *out, const double2 *in, const double2 *U, 
         else
             WD_HOP_F64_LAUNCH(HOP_M, false);
         break;
+    case HOP_M_NORM:
+        if (il)
+            WD_HOP_F64_LAUNCH(HOP_M_NORM, true);
+        else
+            WD_HOP_F64_LAUNCH(HOP_M_NORM, false);
+        break;
+    case HOP_M_AXPY:
+        if (il)
+            WD_HOP_F64_LAUNCH(HOP_M_AXPY, true);
+        else
+            WD_HOP_F64_LAUNCH(HOP_M_AXPY, false);
+        break;
     default:
         if (il)
             WD_HOP_F64_LAUNCH(HOP_M_DOT, true);
@@ -760,6 +822,20 @@ int wd_launch_hop_f64c(wd_ctx *ctx, double2 *out, const double2 *in, F64Links U,
                                   one, 1);
     return launch_links<false>(ctx, out, in, U, cb, mode, a, hc, same, dotv, partials, stop, one,
                                1);
+}
+
+// r = r - alpha (a same - 1/2 hop(in)), HOP_M_AXPY, with the partials of |r|^2;
+// alpha (real) in device memory
+int wd_launch_hop_f64_axpy(wd_ctx *ctx, double2 *r, const double2 *in, F64Links U, int dagger,
+                           int cb, double a, const double2 *same, const double *alpha,
+                           double2 *partials, const int *stop)
+{
+    const F64Batch one = {0, nullptr, nullptr, nullptr, alpha};
+    if (dagger)
+        return launch_links<true>(ctx, r, in, U, cb, HOP_M_AXPY, a, -0.5, same, r, partials, stop,
+                                  one, 1);
+    return launch_links<false>(ctx, r, in, U, cb, HOP_M_AXPY, a, -0.5, same, r, partials, stop,
+                               one, 1);
 }
 
 int wd_launch_hop_f64(wd_ctx *ctx, double2 *out, const double2 *in, F64Links U, int dagger,

@@ -214,6 +214,7 @@ struct F64Batch {
     const int *mask;
     const double *a_rhs;
     const double *c_rhs;
+    const double *axpy; // HOP_M_AXPY: the (real) coefficient alpha, in device memory
 };
 int wd_launch_hop_f64_batch(wd_ctx *ctx, double2 *out, const double2 *in, F64Links U,
                             int dagger, int cb, int mode, const double2 *same,
@@ -234,6 +235,11 @@ enum HopMode {
     HOP_M = 1,        // out[p] = a * same[p] - 1/2 hop(in)  (same == in for M)
     HOP_M_DOT = 2,    // as HOP_M, plus per-block partial of sum conj(dotv) * out
     HOP_M_DOT_NORM = 3, // as HOP_M_DOT, plus per-block partial of |out|^2 (sloppy kernel)
-    HOP_M_NORM = 4,   // as HOP_M, plus per-block partial (|out|^2, 0) (sloppy kernels, no dagger)
-    HOP_M_AXPY = 5,   // out = dotv - axpy_r (M in), partial (|out|^2, 0) (sloppy kernels, dagger only)
+    HOP_M_NORM = 4,   // as HOP_M, plus per-block partial (|out|^2, 0) (sloppy kernels: no dagger)
+    HOP_M_AXPY = 5,   // out = dotv - alpha (M in), partial (|out|^2, 0) (sloppy kernels: dagger only)
 };
+// r -= alpha (a same - 1/2 hop(in)) with the partials of |r|^2 (one system,
+// alpha real, read from device memory)
+int wd_launch_hop_f64_axpy(wd_ctx *ctx, double2 *r, const double2 *in, F64Links U, int dagger,
+                           int cb, double a, const double2 *same, const double *alpha,
+                           double2 *partials, const int *stop);
