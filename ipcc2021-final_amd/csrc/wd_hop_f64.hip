@@ -13,6 +13,12 @@
 //   HOP_M     out[p] = a * same[p] - 1/2 hop(in)          DslashNew, a = 4 + m
 //   HOP_M_DOT HOP_M plus a per-workgroup partial of sum conj(dotv) * out, the
 //             (p, q) product of the CG step fused into q = M^dagger (M p).
+//   HOP_M_NORM HOP_M plus a per-workgroup partial (|out|^2, 0): the CG's
+//             |M p|^2 = (p, M^dagger M p) from the launch that writes M p.
+//   HOP_M_AXPY out = dotv - alpha (a * same - 1/2 hop(in)) (alpha real, read
+//             from device memory; out may be dotv, each site read and written
+//             by its own thread) with the partial (|out|^2, 0): the CG's
+//             r -= alpha M^dagger M p without storing M^dagger M p.
 #include "wd_blas.h"
 #include "wd_dirac.h"
 #include "wd_fields.h"
@@ -468,7 +474,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 1))) 
 }
 
 // t-streaming kernel for the both-parity M / M^dagger launches of one system
-// (HOP_M, HOP_M_DOT).  A thread
+// (HOP_M, HOP_M_DOT, HOP_M_NORM, HOP_M_AXPY).  A thread
 // owns the column (x, y, z) for Tb consecutive t.  Site (x, y, z, t) has
 // parity (x + y + z + t) & 1, so the column's next site is the forward t
 // neighbour: the t link of a site serves its forward hop and, as U^dagger,
